@@ -1,0 +1,176 @@
+/*
+ * mrx_reion.h - C ABI of the MI355X-native excursion-set reionization path.
+ *
+ * Plain C, no globals, no torch or HIP types in any signature.  Every grid is
+ * float32 in the reference's own layouts (src/core/misc_tools.c:200-220):
+ *
+ *   padded real  (INDEX_PADDED)        k + 2(N/2+1)(j + N i)   2*N*N*(N/2+1) floats
+ *   unpadded real (INDEX_REAL)         k + N(j + N i)          N*N*N floats
+ *   half spectrum (INDEX_COMPLEX_HERM) k + (N/2+1)(j + N i)    N*N*(N/2+1) complex
+ *                                                              (interleaved re,im floats)
+ *
+ * Pointers are host or device memory as selected per call; device pointers are
+ * used in place, host pointers are staged through the context's device buffers.
+ * Every entry point returns MRX_OK (0) or a negative MRX_E* code; the message of
+ * the last failure is available from mrx_last_error().  Nothing aborts or throws
+ * across this boundary.  A context is bound to one device and must not be used
+ * from two host threads at once.
+ */
+#ifndef MRX_REION_H
+#define MRX_REION_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRX_OK 0
+#define MRX_EINVAL (-1)       /* bad argument (size, null pointer, flag) */
+#define MRX_ENOMEM (-2)       /* device or host allocation failed */
+#define MRX_EDEVICE (-3)      /* HIP runtime error */
+#define MRX_EUNSUPPORTED (-4) /* grid size or option outside what this build implements */
+#define MRX_EFILTER (-5)      /* invalid filter type (reionization.c:2160-2165 aborts there) */
+
+/* Filter windows of filter() (reionization.c:2110-2170). */
+#define MRX_FILTER_TOPHAT 0 /* real-space top-hat 3(sin x - x cos x)/x^3, x = kR, applied when x > 1e-4 */
+#define MRX_FILTER_SHARPK 1 /* zero the mode when 0.413566994 kR > 1 */
+#define MRX_FILTER_GAUSS 2  /* exp(-(0.643 kR)^2 / 2) */
+
+typedef struct mrx_ctx mrx_ctx;
+
+/*
+ * Knobs of _find_HII_bubbles (find_HII_bubbles.c:47-495).  In the reference they
+ * are read from run_globals.params / .physics / .units; here they are explicit.
+ */
+typedef struct mrx_hii_params {
+  int grid_dim;          /* ReionGridDim (N); the whole box lives on this one GPU          */
+  int filter_type;       /* ReionFilterType, MRX_FILTER_*                                   */
+  int rtom_filter_type;  /* ReionRtoMFilterType: 0 top-hat, 1 Gaussian (RtoM, :26-45)     */
+  int uvb_flag;          /* ReionUVBFlag: write J_21 and J_21_at_ionization                 */
+  int include_spin_temp; /* Flag_IncludeSpinTemp: filter x_e as a 4th field (:144-152)      */
+  int pointers_on_device;/* 1: every grid pointer of the call is device memory             */
+  double box_size;       /* BoxSize [Mpc/h]                                                 */
+  double redshift;       /* ZZ[snapshot]                                                    */
+  double r_bubble_max;   /* ReionRBubbleMax [Mpc/h]                                         */
+  double r_bubble_min;   /* ReionRBubbleMin [Mpc/h]                                         */
+  double delta_r_factor; /* ReionDeltaRFactor                                               */
+  double efficiency;     /* physics.ReionEfficiency                                         */
+  double omega_m;        /* OmegaM                                                          */
+  double rho_crit;       /* run_globals.RhoCrit [internal units] (init.c:204)               */
+  double alpha_uv;       /* physics.ReionAlphaUV                                            */
+  double nion_phot_per_bary; /* physics.ReionNionPhotPerBary                                */
+  double gamma_halo_bias;    /* physics.ReionGammaHaloBias                                  */
+  double unit_length_cm;     /* units.UnitLength_in_cm                                      */
+  double unit_mass_g;        /* units.UnitMass_in_g                                         */
+  double unit_time_s;        /* units.UnitTime_in_s                                         */
+} mrx_hii_params;
+
+/* Scalar results the reference stores in run_globals.reion_grids (:487-494). */
+typedef struct mrx_hii_globals {
+  double volume_weighted_global_xH;
+  double mass_weighted_global_xH;
+  double volume_weighted_global_J_21;
+  int n_radii; /* radius steps taken, including the last unfiltered one */
+} mrx_hii_globals;
+
+/* Knobs of ComputeBrightnessTemperatureBox, spin-temperature branch with
+ * Flag_IncludePecVelsFor21cm = 0 (BrightnessTemperature.c:24-124, :512-545). */
+typedef struct mrx_tb_params {
+  int grid_dim;
+  int include_spin_temp; /* Flag_IncludeSpinTemp: multiply by (1 - T_CMB(1+z)/T_S)          */
+  int pointers_on_device;
+  double redshift;
+  double hubble_h;
+  double omega_m;
+  double baryon_frac;
+} mrx_tb_params;
+
+/* ---------------------------------------------------------------------------
+ * Context.  Replaces init_CUDA() (meraxes_gpu.cu:127-194, declared at
+ * src/meraxes_gpu.h:49-57), which selected the device and kept no buffers; the
+ * context also owns the device work buffers, sized for the largest grid seen,
+ * so repeated snapshots do not reallocate (the CUDA path mallocs per call,
+ * find_HII_bubbles_gpu.cu:151-193).
+ */
+int mrx_init(int device, mrx_ctx** ctx);
+int mrx_free(mrx_ctx* ctx);
+const char* mrx_last_error(const mrx_ctx* ctx);
+/* Synchronise the context's stream (all entry points are synchronous on return
+ * unless stated, so this is only needed after mrx_set_stream with async use). */
+int mrx_synchronize(mrx_ctx* ctx);
+/* Run subsequent work on a caller hipStream_t (NULL: the context's own stream). */
+int mrx_set_stream(mrx_ctx* ctx, void* hip_stream);
+
+/* ---------------------------------------------------------------------------
+ * Radius ladder of the bubble finder (find_HII_bubbles.c:184-207, :445): fills
+ * R_out[0..n) with the radii in Mpc/h, largest first, the last one being the
+ * unfiltered cell-size step, and returns n (or a negative error).  Host only.
+ */
+int mrx_radius_ladder(const mrx_hii_params* p, double* R_out, int max_n);
+
+/* ---------------------------------------------------------------------------
+ * find_HII_bubbles.  Drop-in for _find_HII_bubbles(snapshot)
+ * (find_HII_bubbles.c:47-495) and _find_HII_bubbles_gpu(snapshot, flag)
+ * (find_HII_bubbles_gpu.cu:68-644, declared src/meraxes_gpu.h:53).
+ *
+ * in : deltax, stars, weighted_sfr (padded real); x_e (padded real, only read
+ *      when p->include_spin_temp, may be NULL otherwise)
+ * out: xH, r_bubble (unpadded real, overwritten);
+ *      J_21 (unpadded real, written when p->uvb_flag, may be NULL otherwise)
+ * in/out (state carried across snapshots): z_at_ionization, and
+ *      J_21_at_ionization (only with p->uvb_flag)
+ * globals: volume/mass weighted xH and volume weighted J_21, as the reference.
+ * Inhomogeneous recombinations and minihalos are not implemented.
+ */
+int mrx_find_hii_bubbles(mrx_ctx* ctx, const mrx_hii_params* p, const float* deltax, const float* stars,
+                         const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
+                         float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out);
+
+/* ---------------------------------------------------------------------------
+ * filter().  Drop-in for filter(box, local_ix_start, slab_nx, grid_dim, R,
+ * filter_type) (reionization.c:2110-2170, declared reionization.h:72): multiply
+ * the half spectrum in place by W(|k| R), |k| built with the global x index and
+ * delta_k = 2 pi / box_size.  `box` is INDEX_COMPLEX_HERM, slab_nx planes.
+ */
+int mrx_filter(mrx_ctx* ctx, float* box, int local_ix_start, int slab_nx, int grid_dim, float R, int filter_type,
+               double box_size, int pointers_on_device);
+
+/* ---------------------------------------------------------------------------
+ * 3-D real transforms with FFTW's conventions (unnormalised, forward sign -1),
+ * the operations the reference runs through fftwf_execute on its r2c / c2r
+ * plans (reionization.c:685-736).  in_padded -> out_herm; in_herm -> out_padded
+ * (may alias, as the reference's in-place c2r plans do).
+ */
+int mrx_fft_r2c_3d(mrx_ctx* ctx, int grid_dim, const float* in_padded, float* out_herm, int pointers_on_device);
+int mrx_fft_c2r_3d(mrx_ctx* ctx, int grid_dim, const float* in_herm, float* out_padded, int pointers_on_device);
+
+/* ---------------------------------------------------------------------------
+ * One field through the bubble finder's smoothing chain at radius R: R2C,
+ * divide by N^3 (find_HII_bubbles.c:168-182), filter (skipped when
+ * apply_filter = 0, the last ladder step), C2R, and the aliasing clamp of
+ * :272-311 for `clamp_kind`: 0 density (max(v, -1 + 1e-5)), 1 source grid
+ * (max(v, 0), zero below 1e-8), 2 x_e (as 1, then min(v, 0.999)), -1 none.
+ * Output is padded real.  This exposes the filtered fields the threshold sees.
+ */
+int mrx_filtered_field(mrx_ctx* ctx, int grid_dim, double box_size, int filter_type, const float* field_padded,
+                       double R, int apply_filter, int clamp_kind, float* out_padded, int pointers_on_device);
+
+/* ---------------------------------------------------------------------------
+ * Brightness temperature.  Drop-in for ComputeBrightnessTemperatureBox(snapshot)
+ * (BrightnessTemperature.c:24-124, :512-545; declared BrightnessTemperature.h:9)
+ * for Flag_IncludePecVelsFor21cm = 0.  TS_box is read only with
+ * p->include_spin_temp.  *ave_Tb receives volume_ave_Tb.
+ */
+int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, const float* deltax,
+                        const float* TS_box, float* delta_T, double* ave_Tb);
+
+/* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params),
+ * so a binding can check that its mirror of the layout matches this build. */
+size_t mrx_abi_sizeof(int which);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRX_REION_H */

@@ -1,0 +1,90 @@
+"""Loader of the HIP library libmrx_hip.so (the C ABI of include/mrx_reion.h).
+
+There is no CPU fallback: if the library is missing or cannot load, every
+entry point raises.  Build it with ``python -m meraxes_amd.build``.
+"""
+import ctypes as C
+import os
+
+from .abi import ERRORS, HiiGlobals, HiiParams, TbParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmrx_hip.so")
+
+# every symbol include/mrx_reion.h declares
+EXPORTS = (
+    "mrx_init", "mrx_free", "mrx_last_error", "mrx_synchronize", "mrx_set_stream", "mrx_radius_ladder",
+    "mrx_find_hii_bubbles", "mrx_filter", "mrx_fft_r2c_3d", "mrx_fft_c2r_3d", "mrx_filtered_field",
+    "mrx_brightness_temp", "mrx_abi_sizeof",
+)
+
+_lib = None
+
+
+class MrxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load():
+    """The loaded library; raises OSError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is not built (python -m meraxes_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, f = C.c_void_p, C.c_int, C.c_double, C.c_float
+    ctxp = C.c_void_p
+    L.mrx_init.argtypes = [i, C.POINTER(ctxp)]
+    L.mrx_free.argtypes = [ctxp]
+    L.mrx_last_error.argtypes = [ctxp]
+    L.mrx_last_error.restype = C.c_char_p
+    L.mrx_synchronize.argtypes = [ctxp]
+    L.mrx_set_stream.argtypes = [ctxp, vp]
+    L.mrx_radius_ladder.argtypes = [C.POINTER(HiiParams), vp, i]
+    L.mrx_find_hii_bubbles.argtypes = [ctxp, C.POINTER(HiiParams)] + [vp] * 9 + [C.POINTER(HiiGlobals)]
+    L.mrx_filter.argtypes = [ctxp, vp, i, i, i, f, i, d, i]
+    L.mrx_fft_r2c_3d.argtypes = [ctxp, i, vp, vp, i]
+    L.mrx_fft_c2r_3d.argtypes = [ctxp, i, vp, vp, i]
+    L.mrx_filtered_field.argtypes = [ctxp, i, d, i, vp, d, i, i, vp, i]
+    L.mrx_brightness_temp.argtypes = [ctxp, C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
+    L.mrx_abi_sizeof.argtypes = [i]
+    L.mrx_abi_sizeof.restype = C.c_size_t
+    _lib = L
+    return L
+
+
+class Context:
+    """One mrx_ctx bound to a device (replaces init_CUDA, meraxes_gpu.cu:127-194)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.ptr = C.c_void_p()
+        rc = self.lib.mrx_init(device, C.byref(self.ptr))
+        if rc != 0:
+            raise MrxError(rc, f"mrx_init(device={device}) failed")
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.mrx_last_error(self.ptr)
+            raise MrxError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    def close(self):
+        if self.ptr:
+            self.lib.mrx_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

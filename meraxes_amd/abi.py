@@ -1,0 +1,87 @@
+"""ctypes mirror of include/mrx_reion.h (structs and constants only).
+
+The struct layouts here must match the header field for field; the C-ABI
+tests compare ``ctypes.sizeof`` against ``mrx_abi_sizeof`` exported by the
+library, so a drift fails loudly.
+"""
+import ctypes as C
+
+MRX_OK = 0
+MRX_EINVAL = -1
+MRX_ENOMEM = -2
+MRX_EDEVICE = -3
+MRX_EUNSUPPORTED = -4
+MRX_EFILTER = -5
+
+ERRORS = {
+    MRX_EINVAL: "MRX_EINVAL",
+    MRX_ENOMEM: "MRX_ENOMEM",
+    MRX_EDEVICE: "MRX_EDEVICE",
+    MRX_EUNSUPPORTED: "MRX_EUNSUPPORTED",
+    MRX_EFILTER: "MRX_EFILTER",
+}
+
+FILTER_TOPHAT = 0
+FILTER_SHARPK = 1
+FILTER_GAUSS = 2
+
+
+class HiiParams(C.Structure):
+    """mrx_hii_params: knobs of _find_HII_bubbles (find_HII_bubbles.c:47-495)."""
+
+    _fields_ = [
+        ("grid_dim", C.c_int),
+        ("filter_type", C.c_int),
+        ("rtom_filter_type", C.c_int),
+        ("uvb_flag", C.c_int),
+        ("include_spin_temp", C.c_int),
+        ("pointers_on_device", C.c_int),
+        ("box_size", C.c_double),
+        ("redshift", C.c_double),
+        ("r_bubble_max", C.c_double),
+        ("r_bubble_min", C.c_double),
+        ("delta_r_factor", C.c_double),
+        ("efficiency", C.c_double),
+        ("omega_m", C.c_double),
+        ("rho_crit", C.c_double),
+        ("alpha_uv", C.c_double),
+        ("nion_phot_per_bary", C.c_double),
+        ("gamma_halo_bias", C.c_double),
+        ("unit_length_cm", C.c_double),
+        ("unit_mass_g", C.c_double),
+        ("unit_time_s", C.c_double),
+    ]
+
+
+class HiiGlobals(C.Structure):
+    """mrx_hii_globals: the scalars _find_HII_bubbles leaves in reion_grids."""
+
+    _fields_ = [
+        ("volume_weighted_global_xH", C.c_double),
+        ("mass_weighted_global_xH", C.c_double),
+        ("volume_weighted_global_J_21", C.c_double),
+        ("n_radii", C.c_int),
+    ]
+
+
+class TbParams(C.Structure):
+    """mrx_tb_params: knobs of ComputeBrightnessTemperatureBox."""
+
+    _fields_ = [
+        ("grid_dim", C.c_int),
+        ("include_spin_temp", C.c_int),
+        ("pointers_on_device", C.c_int),
+        ("redshift", C.c_double),
+        ("hubble_h", C.c_double),
+        ("omega_m", C.c_double),
+        ("baryon_frac", C.c_double),
+    ]
+
+
+def fptr(a):
+    """float* of a numpy array or an int device address (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    return a.ctypes.data_as(C.c_void_p)

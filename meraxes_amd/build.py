@@ -1,0 +1,51 @@
+"""Builds libmrx_hip.so in-tree with hipcc for gfx950.
+
+``python -m meraxes_amd.build`` compiles every HIP source of the package.
+Packed-FP32 instructions are disabled (-packed-fp32-ops), and nothing is
+built with fast-math: the window and threshold arithmetic must round exactly
+as the reference's C does.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libmrx_hip.so")
+ARCH = os.environ.get("MRX_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc():
+    for c in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def flags():
+    return [
+        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wno-unused-command-line-argument",
+    ]
+
+
+def build(verbose=False):
+    src = os.path.join(CSRC, "mrx_reion.hip")
+    tmp = OUT + ".tmp"
+    cmd = [hipcc()] + flags() + ["-o", tmp, src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed:\n{err}")
+    if verbose and err:
+        print(err, file=sys.stderr)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

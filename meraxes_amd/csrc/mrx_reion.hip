@@ -1,0 +1,739 @@
+// mrx_reion.hip - host side of the C ABI declared in include/mrx_reion.h.
+//
+// One context per device.  It owns the work buffers (packed spectra U/T of up
+// to four fields, their Nyquist planes V, the per-cell step state, twiddle
+// tables, reduction partials and host-pointer staging), sized for the largest
+// grid seen and kept across calls.  Every entry point validates its arguments
+// on the host, enqueues on the context stream and, unless the caller installed
+// its own stream, synchronises before returning.  Errors come back as MRX_E*
+// codes with a message in mrx_last_error(); nothing aborts.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mrx_reion.h"
+#include "reion_kernels.hpp"
+
+using namespace mrx;
+
+namespace {
+// constants of src/meraxes.h:21-44 and reionization.h:38
+constexpr double kPlanck = 6.6262e-27;
+constexpr double kProtonMass = 1.6726e-24;
+constexpr double kTcmb = 2.728;
+constexpr double kLFactor = 0.620350491;
+constexpr int kMaxSteps = 255;  // the per-cell step index is one byte
+}  // namespace
+
+struct mrx_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  bool user_stream = false;
+  std::string err;
+
+  int N = 0;          // grid the buffers are laid out for
+  int nfield_cap = 0;  // fields U/T/V hold
+  float2* U = nullptr;
+  float2* T = nullptr;
+  float2* V = nullptr;
+  uint8_t* state = nullptr;
+  float2* tw = nullptr;
+  float* Rtab = nullptr;
+  double* partials = nullptr;
+  size_t partials_cap = 0;
+  double* sums = nullptr;
+  // staging for host pointers
+  float* stage = nullptr;
+  size_t stage_cap = 0;
+};
+
+namespace {
+
+int set_err(mrx_ctx* c, int code, const char* fmt, ...)
+{
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIP_TRY(call)                                                                                \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      return set_err(ctx, MRX_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                     __FILE__, __LINE__);                                                            \
+  } while (0)
+
+#define MRX_TRY(expr)          \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != MRX_OK) return rc_; \
+  } while (0)
+
+bool supported_n(int N)
+{
+  switch (N) {
+    case 32:
+    case 48:
+    case 64:
+    case 96:
+    case 128:
+    case 192:
+    case 256:
+    case 384:
+    case 512:
+      return true;
+    default:
+      return false;
+  }
+}
+
+template <class T>
+void dfree(T*& p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int check_launch(mrx_ctx* ctx, const char* what)
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(ctx, MRX_EDEVICE, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return MRX_OK;
+}
+
+int finish(mrx_ctx* ctx)
+{
+  if (!ctx->user_stream) HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
+// Work buffers for an N^3 grid with `nfield` fields.
+int ensure_buffers(mrx_ctx* ctx, int N, int nfield)
+{
+  if (ctx->N == N && ctx->nfield_cap >= nfield) return MRX_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  dfree(ctx->U);
+  dfree(ctx->T);
+  dfree(ctx->V);
+  dfree(ctx->state);
+  dfree(ctx->tw);
+  ctx->N = 0;
+  ctx->nfield_cap = 0;
+  const size_t nc = (size_t)N * N * (N / 2);
+  if (hipMalloc(&ctx->U, sizeof(float2) * nc * nfield) != hipSuccess ||
+      hipMalloc(&ctx->T, sizeof(float2) * nc * nfield) != hipSuccess ||
+      hipMalloc(&ctx->V, sizeof(float2) * (size_t)N * N * nfield) != hipSuccess ||
+      hipMalloc(&ctx->state, (size_t)N * N * N) != hipSuccess || hipMalloc(&ctx->tw, sizeof(float2) * N) != hipSuccess) {
+    (void)hipGetLastError();
+    dfree(ctx->U);
+    dfree(ctx->T);
+    dfree(ctx->V);
+    dfree(ctx->state);
+    dfree(ctx->tw);
+    return set_err(ctx, MRX_ENOMEM, "device allocation for a %d^3 grid with %d fields failed", N, nfield);
+  }
+  std::vector<float2> h(N);
+  for (int t = 0; t < N; ++t) {
+    const double a = -2.0 * M_PI * (double)t / (double)N;
+    h[t] = make_float2((float)cos(a), (float)sin(a));
+  }
+  HIP_TRY(hipMemcpy(ctx->tw, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+  ctx->N = N;
+  ctx->nfield_cap = nfield;
+  return MRX_OK;
+}
+
+int ensure_partials(mrx_ctx* ctx, size_t n)
+{
+  if (ctx->partials_cap >= n) return MRX_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  dfree(ctx->partials);
+  ctx->partials_cap = 0;
+  HIP_TRY(hipMalloc(&ctx->partials, sizeof(double) * n));
+  ctx->partials_cap = n;
+  return MRX_OK;
+}
+
+int ensure_stage(mrx_ctx* ctx, size_t nfloat)
+{
+  if (ctx->stage_cap >= nfloat) return MRX_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  dfree(ctx->stage);
+  ctx->stage_cap = 0;
+  HIP_TRY(hipMalloc(&ctx->stage, sizeof(float) * nfloat));
+  ctx->stage_cap = nfloat;
+  return MRX_OK;
+}
+
+size_t npad(int N) { return (size_t)N * N * (2 * (N / 2 + 1)); }
+size_t nreal(int N) { return (size_t)N * N * N; }
+
+WinArgs win_args(double box_size, float R, int type, int apply)
+{
+  WinArgs w;
+  w.R = R;
+  const float bs = (float)box_size;
+  w.delta_k = (float)(2.0 * M_PI / bs);
+  w.type = type;
+  w.apply = apply;
+  return w;
+}
+
+// ----------------------------------------------------- templated pipelines
+template <int N>
+struct Pipe {
+  static constexpr int NH = N / 2;
+  static constexpr long nc = (long)N * N * NH;
+
+  // padded real fields -> unfiltered packed spectra U (scaled by 1/N^3 when
+  // `scale`), Nyquist planes split into V.
+  static int forward(mrx_ctx* ctx, const float* in, long in_fstride, int nfield, bool scale, float2* U, float2* V)
+  {
+    const dim3 gz(N * N / Cfg<N>::RZ, nfield);
+    k_r2c_z<N><<<gz, kNT, 0, ctx->stream>>>(in, U, ctx->tw, in_fstride, nc);
+    MRX_TRY(check_launch(ctx, "k_r2c_z"));
+    const dim3 gc(N * (NH / Cfg<N>::TW), nfield);
+    // y pass: tile = (x, kz block); rows along y with stride NH
+    k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
+    MRX_TRY(check_launch(ctx, "k_colpass y"));
+    // x pass: tile = (y, kz block); rows along x with stride N*NH
+    const double div = scale ? std::pow((double)N, 3) : 0.0;
+    k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, NH, (long)N * NH, NH / Cfg<N>::TW, div);
+    MRX_TRY(check_launch(ctx, "k_colpass x"));
+    const dim3 gu((N * N + 255) / 256, nfield);
+    k_unpack_dc<N><<<gu, 256, 0, ctx->stream>>>(U, V, nc, (long)N * N);
+    return check_launch(ctx, "k_unpack_dc");
+  }
+
+  // U (+V) filtered at R -> T, then inverse y pass; T then holds the packed
+  // spectra of the filtered fields with x and y already transformed.
+  static int inverse_xy(mrx_ctx* ctx, const float2* U, const float2* V, float2* T, int nfield, const WinArgs& w)
+  {
+    const dim3 gc(N * (NH / Cfg<N>::TW), 1);
+    k_ix_filter<N><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, nc, (long)N * N, nfield, w);
+    MRX_TRY(check_launch(ctx, "k_ix_filter"));
+    const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
+    k_colpass<N, true><<<gy, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
+    return check_launch(ctx, "k_colpass y inv");
+  }
+
+  static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp)
+  {
+    const dim3 gz(N * N / Cfg<N>::RZ, nfield);
+    k_c2r_z<N><<<gz, kNT, 0, ctx->stream>>>(T, out, ctx->tw, nc, (long)npad(N), clamp);
+    return check_launch(ctx, "k_c2r_z");
+  }
+
+  static int threshold(mrx_ctx* ctx, const float* deltax, float* xH, float* rb, float* J21, float* zin, float* J21i,
+                       const StepArgs& a)
+  {
+    const int nb = N * N / Cfg<N>::RZ;
+    k_c2r_threshold<N><<<nb, kNT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, zin, J21i,
+                                                    ctx->Rtab, ctx->partials, a);
+    return check_launch(ctx, "k_c2r_threshold");
+  }
+
+  static int nblocks_threshold() { return N * N / Cfg<N>::RZ; }
+
+  static int pack_herm(mrx_ctx* ctx, const float2* H, float2* U)
+  {
+    const long n = nc;
+    k_herm_to_packed<N><<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(H, U);
+    return check_launch(ctx, "k_herm_to_packed");
+  }
+  static int unpack_herm(mrx_ctx* ctx, const float2* U, const float2* V, float2* H)
+  {
+    const long n = (long)N * N * (NH + 1);
+    k_packed_to_herm<N><<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(U, V, H);
+    return check_launch(ctx, "k_packed_to_herm");
+  }
+  // inverse x then y pass on packed spectra in place (no window)
+  static int inv_xy_plain(mrx_ctx* ctx, float2* T, int nfield)
+  {
+    const dim3 gc(N * (NH / Cfg<N>::TW), nfield);
+    k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, NH, (long)N * NH, NH / Cfg<N>::TW, 0.0);
+    MRX_TRY(check_launch(ctx, "k_colpass x inv"));
+    k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
+    return check_launch(ctx, "k_colpass y inv");
+  }
+};
+
+template <template <int> class F, class... A>
+int dispatch_n(int N, A&&... args)
+{
+  switch (N) {
+    case 32: return F<32>::run(args...);
+    case 48: return F<48>::run(args...);
+    case 64: return F<64>::run(args...);
+    case 96: return F<96>::run(args...);
+    case 128: return F<128>::run(args...);
+    case 192: return F<192>::run(args...);
+    case 256: return F<256>::run(args...);
+    case 384: return F<384>::run(args...);
+    case 512: return F<512>::run(args...);
+    default: return MRX_EUNSUPPORTED;
+  }
+}
+
+double RtoM(const mrx_hii_params* p, double R)
+{
+  // find_HII_bubbles.c:26-45
+  if (p->rtom_filter_type == 0) return (4.0 / 3.0) * M_PI * pow(R, 3) * (p->omega_m * p->rho_crit);
+  return pow(2 * M_PI, 1.5) * p->omega_m * p->rho_crit * pow(R, 3);
+}
+
+int ladder(const mrx_hii_params* p, std::vector<double>& R)
+{
+  // find_HII_bubbles.c:97-100, :184-207, :445
+  const double box_size = p->box_size;
+  const int N = p->grid_dim;
+  double clf = kLFactor;
+  if ((box_size / (double)N) < 1.0) clf = 1.0;
+  double r = fmin(p->r_bubble_max, kLFactor * box_size);
+  R.clear();
+  bool last = false;
+  while (!last) {
+    if (((r / p->delta_r_factor) <= (clf * box_size / (double)N)) || ((r / p->delta_r_factor) <= p->r_bubble_min)) {
+      last = true;
+      r = clf * box_size / (double)N;
+    }
+    R.push_back(r);
+    if (R.size() > 100000) return MRX_EINVAL;
+    r /= p->delta_r_factor;
+  }
+  return (int)R.size();
+}
+
+int validate_hii(mrx_ctx* ctx, const mrx_hii_params* p)
+{
+  if (!p) return set_err(ctx, MRX_EINVAL, "params is NULL");
+  if (!supported_n(p->grid_dim))
+    return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not supported (32,48,64,96,128,192,256,384,512)", p->grid_dim);
+  if (p->filter_type < 0 || p->filter_type > 2)
+    return set_err(ctx, MRX_EFILTER, "ReionFilterType type %d is undefined", p->filter_type);
+  if (p->rtom_filter_type < 0 || p->rtom_filter_type > 1)
+    return set_err(ctx, MRX_EINVAL, "Unrecognised RtoM filter (%d)", p->rtom_filter_type);
+  if (!(p->box_size > 0) || !(p->delta_r_factor > 1.0) || !(p->r_bubble_max > 0))
+    return set_err(ctx, MRX_EINVAL, "box_size, r_bubble_max must be > 0 and delta_r_factor > 1");
+  return MRX_OK;
+}
+
+struct HiiRun {
+  template <int N>
+  struct At {
+    static int run(mrx_ctx* ctx, const mrx_hii_params* p, const float* const* in, float* const* out,
+                   mrx_hii_globals* g)
+    {
+      using P = Pipe<N>;
+      const int nfield = p->include_spin_temp ? 4 : 3;
+      MRX_TRY(ensure_buffers(ctx, N, 4));
+      std::vector<double> R;
+      const int S = ladder(p, R);
+      if (S < 1) return set_err(ctx, MRX_EINVAL, "radius ladder is empty");
+      if (S > kMaxSteps) return set_err(ctx, MRX_EUNSUPPORTED, "radius ladder has %d steps (max %d)", S, kMaxSteps);
+      MRX_TRY(ensure_partials(ctx, (size_t)P::nblocks_threshold() * 4 + 8));
+      if (!ctx->Rtab) HIP_TRY(hipMalloc(&ctx->Rtab, sizeof(float) * 256));
+      if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
+      std::vector<float> Rf(S);
+      for (int s = 0; s < S; ++s) Rf[s] = (float)R[s];
+      HIP_TRY(hipMemcpyAsync(ctx->Rtab, Rf.data(), sizeof(float) * S, hipMemcpyHostToDevice, ctx->stream));
+
+      // inputs: deltax, stars, weighted_sfr, x_e are padded and laid out at a
+      // fixed stride when staged; device pointers are used where they lie.
+      const long fs = (long)npad(N);
+      const float* deltax = in[0];
+      for (int f = 0; f < nfield; ++f) {
+        // forward transform of each field into its U slot
+        MRX_TRY(P::forward(ctx, in[f], 0, 1, true, ctx->U + (long)f * P::nc, ctx->V + (long)f * N * N));
+      }
+      (void)fs;
+      const double box_size = p->box_size;
+      const double pixel_volume = pow(box_size / (double)N, 3);
+      StepArgs a;
+      a.pixel_volume = pixel_volume;
+      a.eff = p->efficiency;
+      a.redshift = p->redshift;
+      a.bias = p->gamma_halo_bias;
+      a.uvb = p->uvb_flag;
+      a.use_xe = p->include_spin_temp;
+      a.nfield = nfield;
+      const double z = p->redshift;
+      for (int s = 0; s < S; ++s) {
+        const bool last = (s == S - 1);
+        const WinArgs w = win_args(box_size, (float)R[s], p->filter_type, last ? 0 : 1);
+        MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, nfield, w));
+        a.J21c = (1.0 + z) * (1.0 + z) / (4.0 * M_PI) * p->alpha_uv * kPlanck * 1e21 * R[s] * p->unit_length_cm *
+                 p->nion_phot_per_bary / kProtonMass * p->unit_mass_g / pow(p->unit_length_cm, 3) / p->unit_time_s;
+        a.M_mean = RtoM(p, R[s]);
+        a.R_cubed = R[s] * R[s] * R[s];
+        a.step = s;
+        a.last = last ? 1 : 0;
+        MRX_TRY(P::threshold(ctx, deltax, out[0], out[1], out[2], out[3], out[4], a));
+      }
+      const int nb = P::nblocks_threshold();
+      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nb, 4, ctx->sums);
+      MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+      double h[4];
+      HIP_TRY(hipMemcpyAsync(h, ctx->sums, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      const double total_n_cells = pow((double)N, 3);
+      if (g) {
+        g->volume_weighted_global_xH = h[0] / total_n_cells;
+        g->mass_weighted_global_xH = h[1] / h[2];
+        g->volume_weighted_global_J_21 = p->uvb_flag ? h[3] / total_n_cells : 0.0;
+        g->n_radii = S;
+      }
+      return MRX_OK;
+    }
+  };
+};
+
+struct FilteredField {
+  template <int N>
+  struct At {
+    static int run(mrx_ctx* ctx, double box_size, int filter_type, const float* in, double R, int apply, int clamp,
+                   float* out)
+    {
+      using P = Pipe<N>;
+      MRX_TRY(ensure_buffers(ctx, N, 1));
+      MRX_TRY(P::forward(ctx, in, 0, 1, true, ctx->U, ctx->V));
+      const WinArgs w = win_args(box_size, (float)R, filter_type, apply);
+      MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
+      return P::c2r_z(ctx, ctx->T, out, 1, clamp);
+    }
+  };
+};
+
+struct R2C {
+  template <int N>
+  struct At {
+    static int run(mrx_ctx* ctx, const float* in, float2* out)
+    {
+      using P = Pipe<N>;
+      MRX_TRY(ensure_buffers(ctx, N, 1));
+      MRX_TRY(P::forward(ctx, in, 0, 1, false, ctx->U, ctx->V));
+      return P::unpack_herm(ctx, ctx->U, ctx->V, out);
+    }
+  };
+};
+
+struct C2R {
+  template <int N>
+  struct At {
+    static int run(mrx_ctx* ctx, const float2* in, float* out)
+    {
+      using P = Pipe<N>;
+      MRX_TRY(ensure_buffers(ctx, N, 1));
+      MRX_TRY(P::pack_herm(ctx, in, ctx->T));
+      MRX_TRY(P::inv_xy_plain(ctx, ctx->T, 1));
+      return P::c2r_z(ctx, ctx->T, out, 1, -1);
+    }
+  };
+};
+
+template <class K>
+struct Disp {
+  template <int N>
+  struct F {
+    template <class... A>
+    static int run(A&&... args)
+    {
+      return K::template At<N>::run(args...);
+    }
+  };
+};
+
+template <class K, class... A>
+int dispatch(int N, A&&... args)
+{
+  return dispatch_n<Disp<K>::template F>(N, args...);
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+extern "C" {
+
+int mrx_init(int device, mrx_ctx** out)
+{
+  if (!out) return MRX_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MRX_EDEVICE;
+  if (device < 0 || device >= ndev) return MRX_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return MRX_EDEVICE;
+  mrx_ctx* ctx = new mrx_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return MRX_EDEVICE;
+  }
+  ctx->stream = ctx->own_stream;
+  *out = ctx;
+  return MRX_OK;
+}
+
+int mrx_free(mrx_ctx* ctx)
+{
+  if (!ctx) return MRX_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  dfree(ctx->U);
+  dfree(ctx->T);
+  dfree(ctx->V);
+  dfree(ctx->state);
+  dfree(ctx->tw);
+  dfree(ctx->Rtab);
+  dfree(ctx->partials);
+  dfree(ctx->sums);
+  dfree(ctx->stage);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return MRX_OK;
+}
+
+const char* mrx_last_error(const mrx_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int mrx_synchronize(mrx_ctx* ctx)
+{
+  if (!ctx) return MRX_EINVAL;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
+int mrx_set_stream(mrx_ctx* ctx, void* s)
+{
+  if (!ctx) return MRX_EINVAL;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  ctx->user_stream = (s != nullptr);
+  return MRX_OK;
+}
+
+int mrx_radius_ladder(const mrx_hii_params* p, double* R_out, int max_n)
+{
+  if (!p || p->grid_dim <= 0 || !(p->delta_r_factor > 1.0)) return MRX_EINVAL;
+  std::vector<double> R;
+  const int n = ladder(p, R);
+  if (n < 0) return n;
+  for (int i = 0; i < n && i < max_n && R_out; ++i) R_out[i] = R[i];
+  return n;
+}
+
+int mrx_find_hii_bubbles(mrx_ctx* ctx, const mrx_hii_params* p, const float* deltax, const float* stars,
+                         const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
+                         float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out)
+{
+  if (!ctx) return MRX_EINVAL;
+  MRX_TRY(validate_hii(ctx, p));
+  if (!deltax || !stars || !weighted_sfr || !xH || !r_bubble || !z_at_ionization)
+    return set_err(ctx, MRX_EINVAL, "a required grid pointer is NULL");
+  if (p->uvb_flag && (!J_21 || !J_21_at_ionization))
+    return set_err(ctx, MRX_EINVAL, "uvb_flag needs J_21 and J_21_at_ionization");
+  if (p->include_spin_temp && !x_e) return set_err(ctx, MRX_EINVAL, "include_spin_temp needs x_e");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int N = p->grid_dim;
+  const size_t np = npad(N), nr = nreal(N);
+  const float* in[4] = {deltax, stars, weighted_sfr, x_e};
+  float* outp[5] = {xH, r_bubble, J_21, z_at_ionization, J_21_at_ionization};
+  if (!p->pointers_on_device) {
+    // stage: 4 padded inputs + 5 unpadded outputs
+    MRX_TRY(ensure_stage(ctx, 4 * np + 5 * nr));
+    float* d_in[4];
+    float* d_out[5];
+    for (int f = 0; f < 4; ++f) d_in[f] = ctx->stage + f * np;
+    for (int o = 0; o < 5; ++o) d_out[o] = ctx->stage + 4 * np + o * nr;
+    const int nfield = p->include_spin_temp ? 4 : 3;
+    for (int f = 0; f < nfield; ++f)
+      HIP_TRY(hipMemcpyAsync(d_in[f], in[f], sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_out[3], z_at_ionization, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+    if (p->uvb_flag)
+      HIP_TRY(hipMemcpyAsync(d_out[4], J_21_at_ionization, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+    const float* din_c[4] = {d_in[0], d_in[1], d_in[2], nfield == 4 ? d_in[3] : nullptr};
+    MRX_TRY(dispatch<HiiRun>(N, ctx, p, din_c, d_out, out));
+    const int nout = p->uvb_flag ? 5 : 4;
+    for (int o = 0; o < nout; ++o) {
+      if (o == 2 && !p->uvb_flag) continue;
+      HIP_TRY(hipMemcpyAsync(outp[o], d_out[o], sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MRX_OK;
+  }
+  MRX_TRY(dispatch<HiiRun>(N, ctx, p, in, outp, out));
+  return finish(ctx);
+}
+
+int mrx_filter(mrx_ctx* ctx, float* box, int local_ix_start, int slab_nx, int grid_dim, float R, int filter_type,
+               double box_size, int on_device)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (filter_type < 0 || filter_type > 2)
+    return set_err(ctx, MRX_EFILTER, "ReionFilterType type %d is undefined", filter_type);
+  if (!box || grid_dim < 2 || (grid_dim & 1) || slab_nx < 0 || local_ix_start < 0 ||
+      local_ix_start + slab_nx > grid_dim || !(box_size > 0))
+    return set_err(ctx, MRX_EINVAL, "bad filter arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const long n = (long)slab_nx * grid_dim * (grid_dim / 2 + 1);
+  if (n == 0) return MRX_OK;
+  float2* d = (float2*)box;
+  if (!on_device) {
+    MRX_TRY(ensure_stage(ctx, 2 * (size_t)n));
+    d = (float2*)ctx->stage;
+    HIP_TRY(hipMemcpyAsync(d, box, sizeof(float2) * n, hipMemcpyHostToDevice, ctx->stream));
+  }
+  const WinArgs w = win_args(box_size, R, filter_type, 1);
+  k_filter_herm<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(d, local_ix_start, slab_nx, grid_dim, w);
+  MRX_TRY(check_launch(ctx, "k_filter_herm"));
+  if (!on_device) {
+    HIP_TRY(hipMemcpyAsync(box, d, sizeof(float2) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MRX_OK;
+  }
+  return finish(ctx);
+}
+
+int mrx_fft_r2c_3d(mrx_ctx* ctx, int N, const float* in_padded, float* out_herm, int on_device)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!supported_n(N)) return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not supported", N);
+  if (!in_padded || !out_herm) return set_err(ctx, MRX_EINVAL, "NULL grid");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t np = npad(N);
+  if (on_device) {
+    MRX_TRY(dispatch<R2C>(N, ctx, in_padded, (float2*)out_herm));
+    return finish(ctx);
+  }
+  MRX_TRY(ensure_stage(ctx, 2 * np));
+  float* din = ctx->stage;
+  float* dout = ctx->stage + np;
+  HIP_TRY(hipMemcpyAsync(din, in_padded, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  MRX_TRY(dispatch<R2C>(N, ctx, (const float*)din, (float2*)dout));
+  HIP_TRY(hipMemcpyAsync(out_herm, dout, sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
+int mrx_fft_c2r_3d(mrx_ctx* ctx, int N, const float* in_herm, float* out_padded, int on_device)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!supported_n(N)) return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not supported", N);
+  if (!in_herm || !out_padded) return set_err(ctx, MRX_EINVAL, "NULL grid");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t np = npad(N);
+  if (on_device) {
+    MRX_TRY(dispatch<C2R>(N, ctx, (const float2*)in_herm, out_padded));
+    return finish(ctx);
+  }
+  MRX_TRY(ensure_stage(ctx, 2 * np));
+  float* din = ctx->stage;
+  float* dout = ctx->stage + np;
+  HIP_TRY(hipMemcpyAsync(din, in_herm, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  MRX_TRY(dispatch<C2R>(N, ctx, (const float2*)din, dout));
+  HIP_TRY(hipMemcpyAsync(out_padded, dout, sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
+int mrx_filtered_field(mrx_ctx* ctx, int N, double box_size, int filter_type, const float* field_padded, double R,
+                       int apply_filter, int clamp_kind, float* out_padded, int on_device)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!supported_n(N)) return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not supported", N);
+  if (filter_type < 0 || filter_type > 2)
+    return set_err(ctx, MRX_EFILTER, "ReionFilterType type %d is undefined", filter_type);
+  if (!field_padded || !out_padded || clamp_kind < -1 || clamp_kind > 2 || !(box_size > 0))
+    return set_err(ctx, MRX_EINVAL, "bad filtered_field arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t np = npad(N);
+  if (on_device) {
+    MRX_TRY(dispatch<FilteredField>(N, ctx, box_size, filter_type, field_padded, R, apply_filter, clamp_kind,
+                                    out_padded));
+    return finish(ctx);
+  }
+  MRX_TRY(ensure_stage(ctx, 2 * np));
+  float* din = ctx->stage;
+  float* dout = ctx->stage + np;
+  HIP_TRY(hipMemcpyAsync(din, field_padded, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  MRX_TRY(dispatch<FilteredField>(N, ctx, box_size, filter_type, (const float*)din, R, apply_filter, clamp_kind,
+                                  dout));
+  HIP_TRY(hipMemcpyAsync(out_padded, dout, sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
+int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, const float* deltax,
+                        const float* TS_box, float* delta_T, double* ave_Tb)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!p || p->grid_dim < 2 || (p->grid_dim & 1) || !xH || !deltax || !delta_T)
+    return set_err(ctx, MRX_EINVAL, "bad brightness_temp arguments");
+  if (p->include_spin_temp && !TS_box) return set_err(ctx, MRX_EINVAL, "include_spin_temp needs TS_box");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int N = p->grid_dim;
+  const size_t np = npad(N), nr = nreal(N);
+  // BrightnessTemperature.c:46-52, :80
+  TbArgs a;
+  const float redshift = (float)(float)p->redshift;
+  const float Hubble_h = (float)(float)p->hubble_h;
+  const float OmegaM = (float)(float)p->omega_m;
+  const float OmegaB = (float)(OmegaM * p->baryon_frac);
+  a.const_factor = (float)(27.0 * (OmegaB * Hubble_h * Hubble_h / 0.023) *
+                           sqrt((0.15 / OmegaM / Hubble_h / Hubble_h) * (1.0 + redshift) / 10.0));
+  a.T_rad = kTcmb * (1. + redshift);
+  a.N = N;
+  a.use_ts = p->include_spin_temp;
+  const float *dxH = xH, *ddx = deltax, *dTS = TS_box;
+  float* ddT = delta_T;
+  if (!p->pointers_on_device) {
+    MRX_TRY(ensure_stage(ctx, np + 3 * nr));
+    float* s = ctx->stage;
+    HIP_TRY(hipMemcpyAsync(s, deltax, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s + np, xH, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+    if (p->include_spin_temp)
+      HIP_TRY(hipMemcpyAsync(s + np + nr, TS_box, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+    ddx = s;
+    dxH = s + np;
+    dTS = p->include_spin_temp ? s + np + nr : nullptr;
+    ddT = s + np + 2 * nr;
+  }
+  const int nb = 2048;
+  MRX_TRY(ensure_partials(ctx, nb + 8));
+  if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
+  k_brightness_temp<<<nb, kNT, 0, ctx->stream>>>(dxH, ddx, dTS, ddT, ctx->partials, (long)nr, a);
+  MRX_TRY(check_launch(ctx, "k_brightness_temp"));
+  k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nb, 1, ctx->sums);
+  MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+  double sum = 0;
+  HIP_TRY(hipMemcpyAsync(&sum, ctx->sums, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (!p->pointers_on_device)
+    HIP_TRY(hipMemcpyAsync(delta_T, ddT, sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ave_Tb) *ave_Tb = sum / pow((double)N, 3);
+  return MRX_OK;
+}
+
+// Size of each ABI struct, so bindings can check their mirrors.
+size_t mrx_abi_sizeof(int which)
+{
+  switch (which) {
+    case 0: return sizeof(mrx_hii_params);
+    case 1: return sizeof(mrx_hii_globals);
+    case 2: return sizeof(mrx_tb_params);
+    default: return 0;
+  }
+}
+
+}  // extern "C"

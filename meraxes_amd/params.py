@@ -1,0 +1,96 @@
+"""Physical parameters and units of the measured configurations.
+
+Values follow the reference's own parameter files: cosmology and units of
+input/params/simulations/Tiamat.par:8-28, reionization knobs of
+input/params/defaults.par:196-254, and the unit derivations of set_units
+(src/core/init.c:184-205).
+"""
+import math
+
+from .abi import HiiParams, TbParams
+
+# src/meraxes.h:21-44
+GRAVITY = 6.672e-8
+SOLAR_MASS = 1.989e33
+SEC_PER_YEAR = 3.155e7
+HUBBLE = 3.2407789e-18
+MPC = 3.086e24
+TCMB = 2.728
+
+# Tiamat.par
+HUBBLE_H = 0.678
+OMEGA_M = 0.308
+OMEGA_L = 0.692
+BARYON_FRAC = 0.15714
+Y_HE = 0.24
+UNIT_LENGTH_CM = 3.08568e24  # Mpc/h
+UNIT_MASS_G = 1.989e43  # 1e10 Msun/h
+UNIT_VELOCITY_CM_S = 1e5  # km/s
+
+
+def units():
+    """UnitTime_in_s, G and RhoCrit as set_units computes them (init.c:184-205)."""
+    unit_time_s = UNIT_LENGTH_CM / UNIT_VELOCITY_CM_S
+    G = GRAVITY / UNIT_LENGTH_CM**3 * UNIT_MASS_G * unit_time_s**2
+    hubble = HUBBLE * unit_time_s
+    rho_crit = 3 * hubble * hubble / (8 * math.pi * G)
+    return {"unit_time_s": unit_time_s, "G": G, "rho_crit": rho_crit}
+
+
+def hii_params(grid_dim, box_size, redshift, filter_type=0, efficiency=20.0, uvb_flag=0,
+               include_spin_temp=0, r_bubble_max=50.0, r_bubble_min=0.4068, delta_r_factor=1.1,
+               rtom_filter_type=0, on_device=0):
+    """mrx_hii_params for one snapshot (defaults.par:221-251 for the unset knobs)."""
+    u = units()
+    p = HiiParams()
+    p.grid_dim = grid_dim
+    p.filter_type = filter_type
+    p.rtom_filter_type = rtom_filter_type
+    p.uvb_flag = uvb_flag
+    p.include_spin_temp = include_spin_temp
+    p.pointers_on_device = on_device
+    p.box_size = box_size
+    p.redshift = redshift
+    p.r_bubble_max = r_bubble_max
+    p.r_bubble_min = r_bubble_min
+    p.delta_r_factor = delta_r_factor
+    p.efficiency = efficiency
+    p.omega_m = OMEGA_M
+    p.rho_crit = u["rho_crit"]
+    p.alpha_uv = 5.0
+    p.nion_phot_per_bary = 4000.0
+    p.gamma_halo_bias = 2.0
+    p.unit_length_cm = UNIT_LENGTH_CM
+    p.unit_mass_g = UNIT_MASS_G
+    p.unit_time_s = u["unit_time_s"]
+    return p
+
+
+def tb_params(grid_dim, redshift, include_spin_temp=0, on_device=0):
+    p = TbParams()
+    p.grid_dim = grid_dim
+    p.include_spin_temp = include_spin_temp
+    p.pointers_on_device = on_device
+    p.redshift = redshift
+    p.hubble_h = HUBBLE_H
+    p.omega_m = OMEGA_M
+    p.baryon_frac = BARYON_FRAC
+    return p
+
+
+def radius_ladder(p):
+    """Radii of the bubble finder, largest first (find_HII_bubbles.c:184-207, :445)."""
+    N = p.grid_dim
+    clf = 0.620350491
+    if p.box_size / N < 1.0:
+        clf = 1.0
+    R = min(p.r_bubble_max, 0.620350491 * p.box_size)
+    out = []
+    while True:
+        last = (R / p.delta_r_factor <= clf * p.box_size / N) or (R / p.delta_r_factor <= p.r_bubble_min)
+        if last:
+            R = clf * p.box_size / N
+        out.append(R)
+        if last:
+            return out
+        R /= p.delta_r_factor

@@ -1,0 +1,170 @@
+"""Host-side mirror of the reference's reionization interface over the C ABI.
+
+Same names and argument meaning as the reference functions each call replaces:
+
+* ``find_hii_bubbles``  -> _find_HII_bubbles (src/core/find_HII_bubbles.c:47-495)
+* ``filter_``           -> filter (src/core/reionization.c:2110-2170)
+* ``fft_r2c_3d`` / ``fft_c2r_3d`` -> the r2c / c2r plans (reionization.c:685-736)
+* ``filtered_field``    -> one field of the smoothing chain (find_HII_bubbles.c:214-311)
+* ``brightness_temp``   -> ComputeBrightnessTemperatureBox (BrightnessTemperature.c:24-549)
+
+Grids are numpy arrays (host memory, staged by the library) or torch tensors
+on the GPU (used in place, on torch's current stream).  Layouts are the
+reference's: padded real (N, N, 2(N/2+1)), unpadded real (N, N, N), half
+spectrum (N, N, N/2+1) complex64.  Invalid arguments raise MrxError with the
+library's message, where the reference aborts (e.g. reionization.c:2160-2165).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Context, MrxError, load  # noqa: F401
+from .abi import HiiGlobals
+
+try:  # torch is optional for host-memory use
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+def _is_dev(a):
+    return torch is not None and isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if _is_dev(a):
+        if not a.is_contiguous():
+            raise ValueError("device grids must be contiguous")
+        return C.c_void_p(a.data_ptr())
+    if not (isinstance(a, np.ndarray) and a.flags.c_contiguous):
+        raise ValueError("host grids must be C-contiguous numpy arrays")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _bind_stream(ctx, on_dev):
+    if on_dev:
+        ctx.check(ctx.lib.mrx_set_stream(ctx.ptr, C.c_void_p(torch.cuda.current_stream().cuda_stream)), "set_stream")
+    else:
+        ctx.check(ctx.lib.mrx_set_stream(ctx.ptr, None), "set_stream")
+
+
+def _empty_like_real(ref, N, fill=None):
+    if _is_dev(ref):
+        t = torch.empty((N, N, N), dtype=torch.float32, device=ref.device)
+        if fill is not None:
+            t.fill_(fill)
+        return t
+    out = np.empty((N, N, N), np.float32)
+    if fill is not None:
+        out.fill(fill)
+    return out
+
+
+def radius_ladder(params):
+    lib = load()
+    buf = np.zeros(4096, np.float64)
+    n = lib.mrx_radius_ladder(C.byref(params), buf.ctypes.data_as(C.c_void_p), 4096)
+    if n < 0:
+        raise MrxError(n, "radius ladder")
+    return buf[:n].copy()
+
+
+def find_hii_bubbles(ctx, params, deltax, stars, weighted_sfr, x_e=None, z_at_ionization=None,
+                     J_21_at_ionization=None, out=None):
+    """One call of the bubble finder; returns the output grids and globals.
+
+    ``z_at_ionization`` / ``J_21_at_ionization`` are the state carried between
+    snapshots (updated in place when given; fresh -1 / 0 grids otherwise).
+    ``out`` may hold preallocated ``xH``, ``r_bubble``, ``J_21`` grids.
+    """
+    N = params.grid_dim
+    on_dev = _is_dev(deltax)
+    params.pointers_on_device = 1 if on_dev else 0
+    out = dict(out or {})
+    xH = out.get("xH") if out.get("xH") is not None else _empty_like_real(deltax, N)
+    rb = out.get("r_bubble") if out.get("r_bubble") is not None else _empty_like_real(deltax, N)
+    J21 = None
+    if params.uvb_flag:
+        J21 = out.get("J_21") if out.get("J_21") is not None else _empty_like_real(deltax, N)
+        if J_21_at_ionization is None:
+            J_21_at_ionization = _empty_like_real(deltax, N, 0.0)
+    if z_at_ionization is None:
+        z_at_ionization = _empty_like_real(deltax, N, -1.0)
+    g = HiiGlobals()
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_find_hii_bubbles(ctx.ptr, C.byref(params), _ptr(deltax), _ptr(stars), _ptr(weighted_sfr),
+                                           _ptr(x_e), _ptr(xH), _ptr(rb), _ptr(J21), _ptr(z_at_ionization),
+                                           _ptr(J_21_at_ionization), C.byref(g)), "find_hii_bubbles")
+    return {
+        "xH": xH, "r_bubble": rb, "J_21": J21, "z_at_ionization": z_at_ionization,
+        "J_21_at_ionization": J_21_at_ionization if params.uvb_flag else None,
+        "volume_weighted_global_xH": g.volume_weighted_global_xH,
+        "mass_weighted_global_xH": g.mass_weighted_global_xH,
+        "volume_weighted_global_J_21": g.volume_weighted_global_J_21, "n_radii": g.n_radii,
+    }
+
+
+def filter_(ctx, box, R, filter_type, box_size, local_ix_start=0):
+    """In-place filter() of a half-spectrum slab (shape (slab_nx, N, N/2+1) complex64)."""
+    on_dev = _is_dev(box)
+    slab_nx, N = box.shape[0], box.shape[1]
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_filter(ctx.ptr, _ptr(box), local_ix_start, slab_nx, N, float(R), filter_type, box_size,
+                                 1 if on_dev else 0), "filter")
+    return box
+
+
+def fft_r2c_3d(ctx, padded, out=None):
+    N = padded.shape[0]
+    on_dev = _is_dev(padded)
+    if out is None:
+        if on_dev:
+            out = torch.empty((N, N, N // 2 + 1), dtype=torch.complex64, device=padded.device)
+        else:
+            out = np.empty((N, N, N // 2 + 1), np.complex64)
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_fft_r2c_3d(ctx.ptr, N, _ptr(padded), _ptr(out), 1 if on_dev else 0), "fft_r2c_3d")
+    return out
+
+
+def fft_c2r_3d(ctx, herm, out=None):
+    N = herm.shape[0]
+    on_dev = _is_dev(herm)
+    if out is None:
+        if on_dev:
+            out = torch.empty((N, N, 2 * (N // 2 + 1)), dtype=torch.float32, device=herm.device)
+        else:
+            out = np.empty((N, N, 2 * (N // 2 + 1)), np.float32)
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_fft_c2r_3d(ctx.ptr, N, _ptr(herm), _ptr(out), 1 if on_dev else 0), "fft_c2r_3d")
+    return out
+
+
+def filtered_field(ctx, field_padded, box_size, filter_type, R, apply_filter=1, clamp_kind=-1, out=None):
+    N = field_padded.shape[0]
+    on_dev = _is_dev(field_padded)
+    if out is None:
+        if on_dev:
+            out = torch.empty_like(field_padded)
+        else:
+            out = np.empty_like(field_padded)
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_filtered_field(ctx.ptr, N, box_size, filter_type, _ptr(field_padded), float(R),
+                                         int(apply_filter), int(clamp_kind), _ptr(out), 1 if on_dev else 0),
+              "filtered_field")
+    return out
+
+
+def brightness_temp(ctx, params, xH, deltax, TS_box=None, out=None):
+    N = params.grid_dim
+    on_dev = _is_dev(xH)
+    params.pointers_on_device = 1 if on_dev else 0
+    if out is None:
+        out = _empty_like_real(xH, N)
+    ave = C.c_double()
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_brightness_temp(ctx.ptr, C.byref(params), _ptr(xH), _ptr(deltax), _ptr(TS_box), _ptr(out),
+                                          C.byref(ave)), "brightness_temp")
+    return out, ave.value

@@ -1,0 +1,174 @@
+"""ctypes front end of the C oracle (liboracle.so, built by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+benchmark's CPU-baseline leg, always as the checker.  The product never
+imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from meraxes_amd.abi import HiiGlobals, HiiParams, TbParams, fptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, i, d, f = C.c_void_p, C.c_int, C.c_double, C.c_float
+        L.orc_find_hii_bubbles.argtypes = [i, C.POINTER(HiiParams)] + [vp] * 9 + [C.POINTER(HiiGlobals), vp]
+        L.orc_filtered_field_f32.argtypes = [i, d, i, vp, d, i, i, vp]
+        L.orc_filtered_field_f64.argtypes = [i, d, i, vp, d, i, i, vp]
+        L.orc_r2c3d_f32.argtypes = [i, vp, vp]
+        L.orc_c2r3d_f32.argtypes = [i, vp, vp]
+        L.orc_r2c3d_f64.argtypes = [i, vp, vp]
+        L.orc_c2r3d_f64.argtypes = [i, vp, vp]
+        L.orc_fft1d_f32.argtypes = [vp, vp, i, i]
+        L.orc_fft1d_f64.argtypes = [vp, vp, i, i]
+        L.orc_filter_f32.argtypes = [vp, i, i, i, f, i, d]
+        L.orc_filter_f64.argtypes = [vp, i, d, i, d]
+        L.orc_window_f32.argtypes = [i, i, i, i, f, i, d]
+        L.orc_window_f32.restype = f
+        L.orc_radius_ladder.argtypes = [C.POINTER(HiiParams), vp, i]
+        L.orc_brightness_temp.argtypes = [C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
+        L.orc_set_threads.argtypes = [i]
+        L.orc_get_threads.restype = i
+        _lib = L
+    return _lib
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with {rc}")
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def fft1d(x, sign=-1):
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.complex64:
+        out = np.empty_like(x)
+        lib().orc_fft1d_f32(fptr(x), fptr(out), x.size, sign)
+    else:
+        x = x.astype(np.complex128)
+        out = np.empty_like(x)
+        lib().orc_fft1d_f64(fptr(x), fptr(out), x.size, sign)
+    return out
+
+
+def r2c3d(padded):
+    """padded real (N,N,2(N/2+1)) -> complex (N,N,N/2+1); float32 or float64."""
+    N = padded.shape[0]
+    if padded.dtype == np.float32:
+        out = np.empty((N, N, N // 2 + 1), np.complex64)
+        lib().orc_r2c3d_f32(N, fptr(np.ascontiguousarray(padded)), fptr(out))
+    else:
+        padded = np.ascontiguousarray(padded, dtype=np.float64)
+        out = np.empty((N, N, N // 2 + 1), np.complex128)
+        lib().orc_r2c3d_f64(N, fptr(padded), fptr(out))
+    return out
+
+
+def c2r3d(herm):
+    N = herm.shape[0]
+    if herm.dtype == np.complex64:
+        out = np.empty((N, N, 2 * (N // 2 + 1)), np.float32)
+        lib().orc_c2r3d_f32(N, fptr(np.ascontiguousarray(herm)), fptr(out))
+    else:
+        herm = np.ascontiguousarray(herm, dtype=np.complex128)
+        out = np.empty((N, N, 2 * (N // 2 + 1)), np.float64)
+        lib().orc_c2r3d_f64(N, fptr(herm), fptr(out))
+    return out
+
+
+def filter_f32(herm, R, filter_type, box_size, local_ix_start=0):
+    herm = np.ascontiguousarray(herm, dtype=np.complex64).copy()
+    N = herm.shape[1]
+    _chk(lib().orc_filter_f32(fptr(herm), local_ix_start, herm.shape[0], N, R, filter_type, box_size), "filter")
+    return herm
+
+
+def filter_f64(herm, R, filter_type, box_size):
+    herm = np.ascontiguousarray(herm, dtype=np.complex128).copy()
+    _chk(lib().orc_filter_f64(fptr(herm), herm.shape[0], R, filter_type, box_size), "filter64")
+    return herm
+
+
+def window_f32(nx, ny, nz, N, R, filter_type, box_size):
+    return lib().orc_window_f32(nx, ny, nz, N, R, filter_type, box_size)
+
+
+def radius_ladder(p):
+    buf = np.zeros(4096, np.float64)
+    n = lib().orc_radius_ladder(C.byref(p), fptr(buf), 4096)
+    _chk(0 if n > 0 else n, "ladder")
+    return buf[:n].copy()
+
+
+def filtered_field(N, box_size, filter_type, field_padded, R, apply_filter=1, clamp_kind=-1, precision=32):
+    field_padded = np.ascontiguousarray(field_padded, dtype=np.float32)
+    if precision == 64:
+        out = np.empty((N, N, 2 * (N // 2 + 1)), np.float64)
+        rc = lib().orc_filtered_field_f64(N, box_size, filter_type, fptr(field_padded), R, apply_filter,
+                                          clamp_kind, fptr(out))
+    else:
+        out = np.empty((N, N, 2 * (N // 2 + 1)), np.float32)
+        rc = lib().orc_filtered_field_f32(N, box_size, filter_type, fptr(field_padded), R, apply_filter,
+                                          clamp_kind, fptr(out))
+    _chk(rc, "filtered_field")
+    return out
+
+
+def find_hii_bubbles(p, deltax, stars, wsfr, x_e=None, z_at_ion=None, J21_at_ion=None, precision=32,
+                     want_ratio=False):
+    """Runs the restated _find_HII_bubbles; returns a dict of unpadded outputs."""
+    N = p.grid_dim
+    shp = (N, N, N)
+    xH = np.empty(shp, np.float32)
+    rb = np.empty(shp, np.float32)
+    J21 = np.empty(shp, np.float32) if p.uvb_flag else None
+    z_in = (np.full(shp, -1.0, np.float32) if z_at_ion is None else np.array(z_at_ion, np.float32, copy=True))
+    J21a = None
+    if p.uvb_flag:
+        J21a = np.zeros(shp, np.float32) if J21_at_ion is None else np.array(J21_at_ion, np.float32, copy=True)
+    ratio = np.empty(shp, np.float32) if want_ratio else None
+    g = HiiGlobals()
+    args = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None for a in (deltax, stars, wsfr, x_e)]
+    rc = lib().orc_find_hii_bubbles(precision, C.byref(p), *[fptr(a) for a in args], fptr(xH), fptr(rb), fptr(J21),
+                                    fptr(z_in), fptr(J21a), C.byref(g), fptr(ratio))
+    _chk(rc, "find_hii_bubbles")
+    return {
+        "xH": xH, "r_bubble": rb, "J_21": J21, "z_at_ionization": z_in, "J_21_at_ionization": J21a,
+        "max_ratio": ratio, "volume_weighted_global_xH": g.volume_weighted_global_xH,
+        "mass_weighted_global_xH": g.mass_weighted_global_xH,
+        "volume_weighted_global_J_21": g.volume_weighted_global_J_21, "n_radii": g.n_radii,
+    }
+
+
+def brightness_temp(p, xH, deltax, TS_box=None):
+    N = p.grid_dim
+    out = np.empty((N, N, N), np.float32)
+    ave = C.c_double()
+    args = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None for a in (xH, deltax, TS_box)]
+    _chk(lib().orc_brightness_temp(C.byref(p), *[fptr(a) for a in args], fptr(out), C.byref(ave)), "tb")
+    return out, ave.value
