@@ -1,0 +1,191 @@
+"""Benchmark of the excursion-set reionization path on one MI355X.
+
+A step is one snapshot of the hot path on a 512^3, 400 Mpc/h box at z = 8:
+find_HII_bubbles over the full radius ladder (44 radii, top-hat window, UVB on,
+three fields filtered per radius) followed by the brightness-temperature grid,
+all inputs resident in HBM when the timed region starts.  The metric is
+cells*radii per second (N^3 * number of radii / step time).
+
+Besides the headline line it reports, from HIP events recorded around every
+kernel launch on the stream the kernels run on, the dominant kernel's average
+launch time and its achieved HBM bandwidth against the MI355X peak, the
+whole-path fraction of peak on the 3-pass byte model, and a CPU baseline: the
+oracle (the C restatement of the reference's FFTW-float path) on the host cores
+over a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def model_bytes_hii(N, F, S, uvb):
+    """3-pass algorithmic byte model of the bubble finder (bytes per snapshot)."""
+    Bc = 8 * N * N * (N // 2 + 1)
+    Nr = N**3
+    W = 20 + (8 if uvb else 0)
+    return F * 6 * Bc + S * (F * 6 * Bc + (4 * F + W) * Nr)
+
+
+def kernel_bytes(name, N, F):
+    """Algorithmic HBM bytes of one launch of a kernel (packed spectra: 8*N*N*N/2 bytes)."""
+    spec = 8 * N * N * (N // 2)
+    Nr = N**3
+    table = {
+        # strided pencil passes: read + write one spectrum per field
+        "colpass_y_inv": 2 * F * spec,
+        "colpass_y_fwd": 2 * spec,
+        "colpass_x_fwd": 2 * spec,
+        # window + inverse x pass: read unfiltered (+ Nyquist plane) and write per field
+        "ix_filter": F * (2 * spec + 8 * N * N),
+        # inverse z pass + threshold: read each field's spectrum, 1 byte of step state per cell
+        "c2r_threshold": F * spec + 2 * Nr,
+        "r2c_z": 4 * N * N * (N + 2) + spec,
+        "brightness_temp": 4 * N * N * (N + 2) + 2 * 4 * Nr,
+    }
+    return table.get(name)
+
+
+def cpu_baseline(sample_grid=256, box=200.0):
+    """Oracle (ref32) over the full ladder of a 256^3, 200 Mpc/h box (same cell size as 512^3/400)."""
+    from meraxes_amd import params as P
+    from meraxes_amd import synth
+    from oracle import oracle as O
+
+    d, s, w = synth.snapshot_inputs(sample_grid, 1)
+    p = P.hii_params(sample_grid, box, 8.0, filter_type=0, efficiency=20.0, uvb_flag=1)
+    t0 = time.perf_counter()
+    r = O.find_hii_bubbles(p, d, s, w)
+    dt = time.perf_counter() - t0
+    S = r["n_radii"]
+    return {
+        "value": sample_grid**3 * S / dt,
+        "unit": "cells*radii/s",
+        "cores": O.get_threads(),
+        "kind": "port",
+        "sample": f"oracle ref32 find_HII_bubbles, {sample_grid}^3 box {box} Mpc/h z=8, full ladder S={S}, "
+                  f"F=3, UVB on, top-hat; {dt:.1f} s wall",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the reionization path runs the whole box on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from meraxes_amd import params as P
+    from meraxes_amd import reion, synth
+    from meraxes_amd._lib import Context
+
+    N = args.grid
+    L = 400.0 * N / 512
+    z = 8.0
+    log(f"[bench] generating {N}^3 inputs")
+    d, s, w = synth.snapshot_inputs(N, 0)
+    dev = torch.device("cuda:0")
+    dd, ds, dw = (torch.from_numpy(a).to(dev) for a in (d, s, w))
+    del d, s, w
+    p = P.hii_params(N, L, z, filter_type=0, efficiency=20.0, uvb_flag=1)
+    tp = P.tb_params(N, z)
+    S = len(P.radius_ladder(p))
+    F = 3
+    xH = torch.empty((N, N, N), dtype=torch.float32, device=dev)
+    rb = torch.empty_like(xH)
+    J21 = torch.empty_like(xH)
+    dT = torch.empty_like(xH)
+    zin = torch.full_like(xH, -1.0)
+    J21i = torch.zeros_like(xH)
+    ctx = Context(0)
+
+    def step():
+        zin.fill_(-1.0)
+        r = reion.find_hii_bubbles(ctx, p, dd, ds, dw, z_at_ionization=zin, J_21_at_ionization=J21i,
+                                   out={"xH": xH, "r_bubble": rb, "J_21": J21})
+        reion.brightness_temp(ctx, tp, xH, dd, out=dT)
+        return r
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        r = step()
+    torch.cuda.synchronize()
+    reion.profile(ctx, True)
+    log(f"[bench] timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    prof = reion.profile_read(ctx)
+    reion.profile(ctx, False)
+    log(f"[bench] <xH>_V = {r['volume_weighted_global_xH']:.6f}, {dt * 1e3:.2f} ms/step")
+    for k, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        log(f"[bench]   {k:18s} {ms / args.steps:9.3f} ms/step  {n // args.steps:5d} launches/step  "
+            f"{ms / n * 1e3:9.1f} us/launch")
+
+    # dominant kernel roofline from the event timings
+    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    name, (ms_tot, cnt) = dom
+    avg_s = ms_tot / cnt / 1e3
+    b = kernel_bytes(name, N, F)
+    roof = {"kernel": name, "bound": "hbm", "achieved": (b / avg_s / 1e9) if b else None, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": (b / avg_s / HBM_PEAK) if b else None, "traffic": None,
+            "avg_launch_us": avg_s * 1e6, "bytes_per_launch": b}
+    mb = model_bytes_hii(N, F, S, True) + 16 * N**3
+    cells_radii = N**3 * S
+    out = {
+        "metric": "cells*radii/s of find_HII_bubbles + dTb at 512^3 (plain path)",
+        "value": cells_radii / dt,
+        "unit": "cells*radii/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"find_HII_bubbles+dTb, {N}^3 grid, {L:g} Mpc/h, z={z}, top-hat, UVB on, "
+                               f"eps=20, {S} radii, {F} fields", "grid": N, "box_mpc_h": L, "radii": S,
+                   "fields": F, "parallelism": "single GPU"},
+        "model_hbm_frac": mb / dt / HBM_PEAK,
+        "model_bytes": mb,
+        "roofline": roof,
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline()
+        log(f"[bench] CPU baseline {out['cpu_baseline']['value']:.3e} cells*radii/s "
+            f"on {out['cpu_baseline']['cores']} threads")
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

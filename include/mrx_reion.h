@@ -86,6 +86,47 @@ typedef struct mrx_tb_params {
   double baryon_frac;
 } mrx_tb_params;
 
+/*
+ * Knobs of _ComputeTs (src/core/ComputeTs.c:78-1020) and the X-ray / Ly-alpha
+ * physics it calls (src/core/XRayHeatingFunctions.c); minihalos, Lyman-Werner
+ * and peculiar velocities off.
+ */
+typedef struct mrx_ts_params {
+  int grid_dim;
+  int pointers_on_device;
+  int num_filter_steps;    /* TsNumFilterSteps (<= 64)                                   */
+  int heating_filter_type; /* TsHeatingFilterType, MRX_FILTER_*                          */
+  int rtom_filter_type;    /* ReionRtoMFilterType (collapse fraction of the first shell) */
+  int snapshot;            /* index of this snapshot in redshifts[]                      */
+  int n_history;           /* SFR history grids held (run_globals.NstoreSnapshots_SFR)   */
+  int reproduce_quirks;    /* 1: volume_ave_xe as the reference computes it (ComputeTs.c:957) */
+  double box_size;         /* BoxSize [Mpc/h]                                            */
+  double hubble_h, omega_m, omega_lambda, omega_r, omega_k, w_lambda, baryon_frac, y_he;
+  double l_xray_gal;            /* LXrayGal [erg/s per Msun/yr]                          */
+  double nu_xray_gal_threshold; /* NuXrayGalThreshold [eV]                               */
+  double spec_index_xray_gal;   /* SpecIndexXrayGal                                      */
+  double nu_xray_soft_cut;      /* NuXraySoftCut [eV]                                    */
+  double nu_xray_max;           /* NuXrayMax [eV]                                        */
+  double reion_efficiency;      /* ReionEfficiency                                       */
+  double nion_phot_per_bary;    /* ReionNionPhotPerBary (Pop II Ly-series normalisation) */
+  double max_heating_redshift;  /* ReionMaxHeatingRedshift                               */
+  double rho_crit;              /* run_globals.RhoCrit [internal units]                  */
+  double unit_length_cm, unit_mass_g, unit_time_s;
+  const double* redshifts; /* ZZ[0..snapshot]                                            */
+  double* stored_fcoll;    /* collapse-fraction history, entry [snapshot] is written      */
+  const char* tables_dir;  /* directory of the 21cmFAST tables (TablesForXHeatingDir)    */
+} mrx_ts_params;
+
+/* Scalar results _ComputeTs leaves in run_globals.reion_grids (:940-985). */
+typedef struct mrx_ts_globals {
+  double volume_ave_J_alpha, volume_ave_xalpha, volume_ave_Xheat, volume_ave_Xion;
+  double volume_ave_TS, volume_ave_TK, volume_ave_xe;
+  double collapse_fraction; /* mean collapse fraction of the first shell           */
+  double x_e_ave;           /* <x_e> entering the shell tables                      */
+  int no_light;             /* 1 when collapse_fraction == 0 (no sources yet)        */
+  int homogeneous;          /* 1 when zp >= ReionMaxHeatingRedshift (RECFAST branch) */
+} mrx_ts_globals;
+
 /* ---------------------------------------------------------------------------
  * Context.  Replaces init_CUDA() (meraxes_gpu.cu:127-194, declared at
  * src/meraxes_gpu.h:49-57), which selected the device and kept no buffers; the
@@ -157,6 +198,22 @@ int mrx_filtered_field(mrx_ctx* ctx, int grid_dim, double box_size, int filter_t
                        double R, int apply_filter, int clamp_kind, float* out_padded, int pointers_on_device);
 
 /* ---------------------------------------------------------------------------
+ * Spin temperature.  Drop-in for _ComputeTs(snapshot) (ComputeTs.c:78-1020),
+ * dispatched where ComputeTs() (:1024-1045) calls it.
+ *
+ * in : deltax, stars, sfr (padded real, this snapshot); sfr_histories:
+ *      p->n_history grids of 2*N*N*(N/2+1) floats each, slot k = k snapshots
+ *      back, stored and read exactly as the reference's sfr_histories buffer
+ *      (written with INDEX_PADDED, read with INDEX_REAL, reionization.c:1586,
+ *      :1767).  Only read when the shells reach back in time.
+ * in/out: x_e_box_prev (padded), Tk_box (unpadded) - the carried state.
+ * out: x_e_box (padded, = the new x_e_box_prev), TS_box (unpadded).
+ */
+int mrx_compute_ts(mrx_ctx* ctx, const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
+                   const float* sfr_histories, float* x_e_box, float* x_e_box_prev, float* Tk_box, float* TS_box,
+                   mrx_ts_globals* out);
+
+/* ---------------------------------------------------------------------------
  * Brightness temperature.  Drop-in for ComputeBrightnessTemperatureBox(snapshot)
  * (BrightnessTemperature.c:24-124, :512-545; declared BrightnessTemperature.h:9)
  * for Flag_IncludePecVelsFor21cm = 0.  TS_box is read only with
@@ -165,7 +222,20 @@ int mrx_filtered_field(mrx_ctx* ctx, int grid_dim, double box_size, int filter_t
 int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, const float* deltax,
                         const float* TS_box, float* delta_T, double* ave_Tb);
 
-/* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params),
+/* ---------------------------------------------------------------------------
+ * Kernel timing, the device-side counterpart of the reference's timer_info
+ * (src/core/utils.c:205-220) around find_HII_bubbles / ComputeTs
+ * (find_HII_bubbles.c:498-531).  While enabled, every kernel launch is
+ * bracketed by HIP events on the stream it runs on.  mrx_profile_read()
+ * synchronises, writes up to max_kinds kernel names (each name_len bytes into
+ * `names`), their summed milliseconds and launch counts, clears the record and
+ * returns the number of distinct kernels.
+ */
+int mrx_profile(mrx_ctx* ctx, int enable);
+int mrx_profile_read(mrx_ctx* ctx, int max_kinds, char* names, int name_len, double* ms_total, long* counts);
+
+/* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params,
+ * 3 mrx_ts_params, 4 mrx_ts_globals),
  * so a binding can check that its mirror of the layout matches this build. */
 size_t mrx_abi_sizeof(int which);
 

@@ -6,7 +6,7 @@ entry point raises.  Build it with ``python -m meraxes_amd.build``.
 import ctypes as C
 import os
 
-from .abi import ERRORS, HiiGlobals, HiiParams, TbParams
+from .abi import ERRORS, HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmrx_hip.so")
@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmrx_hip.so")
 EXPORTS = (
     "mrx_init", "mrx_free", "mrx_last_error", "mrx_synchronize", "mrx_set_stream", "mrx_radius_ladder",
     "mrx_find_hii_bubbles", "mrx_filter", "mrx_fft_r2c_3d", "mrx_fft_c2r_3d", "mrx_filtered_field",
-    "mrx_brightness_temp", "mrx_abi_sizeof",
+    "mrx_brightness_temp", "mrx_abi_sizeof", "mrx_profile", "mrx_profile_read", "mrx_compute_ts",
 )
 
 _lib = None
@@ -50,6 +50,9 @@ def load():
     L.mrx_fft_c2r_3d.argtypes = [ctxp, i, vp, vp, i]
     L.mrx_filtered_field.argtypes = [ctxp, i, d, i, vp, d, i, i, vp, i]
     L.mrx_brightness_temp.argtypes = [ctxp, C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
+    L.mrx_compute_ts.argtypes = [ctxp, C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals)]
+    L.mrx_profile.argtypes = [ctxp, i]
+    L.mrx_profile_read.argtypes = [ctxp, i, C.c_char_p, i, vp, vp]
     L.mrx_abi_sizeof.argtypes = [i]
     L.mrx_abi_sizeof.restype = C.c_size_t
     _lib = L

@@ -78,6 +78,63 @@ class TbParams(C.Structure):
     ]
 
 
+class TsParams(C.Structure):
+    """mrx_ts_params: knobs of _ComputeTs (ComputeTs.c:78-1020)."""
+
+    _fields_ = [
+        ("grid_dim", C.c_int),
+        ("pointers_on_device", C.c_int),
+        ("num_filter_steps", C.c_int),
+        ("heating_filter_type", C.c_int),
+        ("rtom_filter_type", C.c_int),
+        ("snapshot", C.c_int),
+        ("n_history", C.c_int),
+        ("reproduce_quirks", C.c_int),
+        ("box_size", C.c_double),
+        ("hubble_h", C.c_double),
+        ("omega_m", C.c_double),
+        ("omega_lambda", C.c_double),
+        ("omega_r", C.c_double),
+        ("omega_k", C.c_double),
+        ("w_lambda", C.c_double),
+        ("baryon_frac", C.c_double),
+        ("y_he", C.c_double),
+        ("l_xray_gal", C.c_double),
+        ("nu_xray_gal_threshold", C.c_double),
+        ("spec_index_xray_gal", C.c_double),
+        ("nu_xray_soft_cut", C.c_double),
+        ("nu_xray_max", C.c_double),
+        ("reion_efficiency", C.c_double),
+        ("nion_phot_per_bary", C.c_double),
+        ("max_heating_redshift", C.c_double),
+        ("rho_crit", C.c_double),
+        ("unit_length_cm", C.c_double),
+        ("unit_mass_g", C.c_double),
+        ("unit_time_s", C.c_double),
+        ("redshifts", C.POINTER(C.c_double)),
+        ("stored_fcoll", C.POINTER(C.c_double)),
+        ("tables_dir", C.c_char_p),
+    ]
+
+
+class TsGlobals(C.Structure):
+    """mrx_ts_globals: the scalars _ComputeTs leaves in reion_grids."""
+
+    _fields_ = [
+        ("volume_ave_J_alpha", C.c_double),
+        ("volume_ave_xalpha", C.c_double),
+        ("volume_ave_Xheat", C.c_double),
+        ("volume_ave_Xion", C.c_double),
+        ("volume_ave_TS", C.c_double),
+        ("volume_ave_TK", C.c_double),
+        ("volume_ave_xe", C.c_double),
+        ("collapse_fraction", C.c_double),
+        ("x_e_ave", C.c_double),
+        ("no_light", C.c_int),
+        ("homogeneous", C.c_int),
+    ]
+
+
 def fptr(a):
     """float* of a numpy array or an int device address (None -> NULL)."""
     if a is None:

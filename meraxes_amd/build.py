@@ -34,7 +34,18 @@ def flags():
 def build(verbose=False):
     src = os.path.join(CSRC, "mrx_reion.hip")
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + flags() + ["-o", tmp, src]
+    # host-side spin-temperature tables: plain C, no fused multiply-adds
+    obj = os.path.join(CSRC, "ts_tables.o")
+    ccmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall", "-c",
+            os.path.join(CSRC, "ts_tables.c"), "-o", obj]
+    if verbose:
+        print(" ".join(ccmd), flush=True)
+    r = subprocess.run(ccmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed:\n{r.stderr}")
+    # the HIP translation unit and the C object are linked together through
+    # the linker so hipcc does not treat the object as a HIP source
+    cmd = [hipcc()] + flags() + ["-o", tmp, src, "-Xlinker", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -18,6 +18,8 @@
 
 #include "../../include/mrx_reion.h"
 #include "reion_kernels.hpp"
+#include "ts_kernels.hpp"
+#include "ts_tables.h"
 
 using namespace mrx;
 
@@ -51,6 +53,24 @@ struct mrx_ctx {
   // staging for host pointers
   float* stage = nullptr;
   size_t stage_cap = 0;
+  // kernel timing (mrx_profile): events recorded around each launch
+  struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  bool prof = false;
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> event_pool;
+  // spin temperature: host tables (reloaded when the key changes) and device buffers
+  mrx_ts_data* tsd = nullptr;
+  std::string ts_key;
+  mrx_ts_snap* tss = nullptr;
+  TsDev* ts_host = nullptr;
+  TsDev* ts_dev = nullptr;
+  float* ts_work = nullptr;
+  size_t ts_work_cap = 0;
+  float* ts_shells = nullptr;
+  size_t ts_shells_cap = 0;
 };
 
 namespace {
@@ -178,6 +198,57 @@ int ensure_stage(mrx_ctx* ctx, size_t nfloat)
   return MRX_OK;
 }
 
+hipEvent_t take_event(mrx_ctx* ctx)
+{
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one kernel launch with events on the launch stream when profiling.
+struct ProfScope {
+  mrx_ctx* ctx;
+  const char* name;
+  hipEvent_t a = nullptr;
+  ProfScope(mrx_ctx* c, const char* n) : ctx(c), name(n)
+  {
+    if (ctx->prof) {
+      a = take_event(ctx);
+      if (a) (void)hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~ProfScope()
+  {
+    if (ctx->prof && a) {
+      hipEvent_t b = take_event(ctx);
+      if (b) {
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->prof_recs.push_back({name, a, b});
+      }
+    }
+  }
+};
+#define PROF(name) ProfScope prof_scope_##__LINE__(ctx, name)
+
+int ensure_buf(mrx_ctx* ctx, float*& p, size_t& cap, size_t nfloat)
+{
+  if (cap >= nfloat) return MRX_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  dfree(p);
+  cap = 0;
+  if (hipMalloc(&p, sizeof(float) * nfloat) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_err(ctx, MRX_ENOMEM, "device allocation of %zu floats failed", nfloat);
+  }
+  cap = nfloat;
+  return MRX_OK;
+}
+
 size_t npad(int N) { return (size_t)N * N * (2 * (N / 2 + 1)); }
 size_t nreal(int N) { return (size_t)N * N * N; }
 
@@ -200,20 +271,32 @@ struct Pipe {
 
   // padded real fields -> unfiltered packed spectra U (scaled by 1/N^3 when
   // `scale`), Nyquist planes split into V.
-  static int forward(mrx_ctx* ctx, const float* in, long in_fstride, int nfield, bool scale, float2* U, float2* V)
+  static int forward(mrx_ctx* ctx, const float* in, long in_fstride, int nfield, bool scale, float2* U, float2* V,
+                     bool float_scale = false)
   {
     const dim3 gz(N * N / Cfg<N>::RZ, nfield);
+    {
+    PROF("r2c_z");
     k_r2c_z<N><<<gz, kNT, 0, ctx->stream>>>(in, U, ctx->tw, in_fstride, nc);
+    }
     MRX_TRY(check_launch(ctx, "k_r2c_z"));
     const dim3 gc(N * (NH / Cfg<N>::TW), nfield);
     // y pass: tile = (x, kz block); rows along y with stride NH
+    {
+    PROF("colpass_y_fwd");
     k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
+    }
     MRX_TRY(check_launch(ctx, "k_colpass y"));
     // x pass: tile = (y, kz block); rows along x with stride N*NH
-    const double div = scale ? std::pow((double)N, 3) : 0.0;
+    double div = scale ? std::pow((double)N, 3) : 0.0;
+    if (scale && float_scale) div = -div;
+    {
+    PROF("colpass_x_fwd");
     k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, NH, (long)N * NH, NH / Cfg<N>::TW, div);
+    }
     MRX_TRY(check_launch(ctx, "k_colpass x"));
     const dim3 gu((N * N + 255) / 256, nfield);
+    PROF("unpack_dc");
     k_unpack_dc<N><<<gu, 256, 0, ctx->stream>>>(U, V, nc, (long)N * N);
     return check_launch(ctx, "k_unpack_dc");
   }
@@ -223,17 +306,22 @@ struct Pipe {
   static int inverse_xy(mrx_ctx* ctx, const float2* U, const float2* V, float2* T, int nfield, const WinArgs& w)
   {
     const dim3 gc(N * (NH / Cfg<N>::TW), 1);
+    {
+    PROF("ix_filter");
     k_ix_filter<N><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, nc, (long)N * N, nfield, w);
+    }
     MRX_TRY(check_launch(ctx, "k_ix_filter"));
     const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
+    PROF("colpass_y_inv");
     k_colpass<N, true><<<gy, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
     return check_launch(ctx, "k_colpass y inv");
   }
 
-  static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp)
+  static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp, int pitch = N + 2)
   {
     const dim3 gz(N * N / Cfg<N>::RZ, nfield);
-    k_c2r_z<N><<<gz, kNT, 0, ctx->stream>>>(T, out, ctx->tw, nc, (long)npad(N), clamp);
+    PROF("c2r_z");
+    k_c2r_z<N><<<gz, kNT, 0, ctx->stream>>>(T, out, ctx->tw, nc, (long)N * N * pitch, clamp, pitch);
     return check_launch(ctx, "k_c2r_z");
   }
 
@@ -241,6 +329,7 @@ struct Pipe {
                        const StepArgs& a)
   {
     const int nb = N * N / Cfg<N>::RZ;
+    PROF("c2r_threshold");
     k_c2r_threshold<N><<<nb, kNT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, zin, J21i,
                                                     ctx->Rtab, ctx->partials, a);
     return check_launch(ctx, "k_c2r_threshold");
@@ -384,7 +473,10 @@ struct HiiRun {
         MRX_TRY(P::threshold(ctx, deltax, out[0], out[1], out[2], out[3], out[4], a));
       }
       const int nb = P::nblocks_threshold();
+      {
+      PROF("reduce_partials");
       k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nb, 4, ctx->sums);
+      }
       MRX_TRY(check_launch(ctx, "k_reduce_partials"));
       double h[4];
       HIP_TRY(hipMemcpyAsync(h, ctx->sums, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
@@ -440,6 +532,214 @@ struct C2R {
       MRX_TRY(P::pack_herm(ctx, in, ctx->T));
       MRX_TRY(P::inv_xy_plain(ctx, ctx->T, 1));
       return P::c2r_z(ctx, ctx->T, out, 1, -1);
+    }
+  };
+};
+
+
+// ------------------------------------------------------------ spin temperature
+void fill_kappa(KappaSpline& k, const mrx_spline& s)
+{
+  k.n = s.n;
+  for (int i = 0; i < s.n && i < 27; ++i) {
+    k.x[i] = s.x[i];
+    k.y[i] = s.y[i];
+    k.b[i] = s.b[i];
+    k.c[i] = s.c[i];
+    k.d[i] = s.d[i];
+  }
+}
+
+// Snapshot constants of the per-cell kernel, each the prefix the reference
+// evaluates first (see TsDev).
+void fill_ts_dev(TsDev& t, const mrx_ts_data* d, const mrx_ts_snap* s, const mrx_ts_params* p)
+{
+  memset(&t, 0, sizeof t);
+  const int N = p->grid_dim;
+  const double box_size = p->box_size / p->hubble_h;
+  t.nsteps = s->nsteps;
+  t.no_light = s->no_light;
+  t.pixel_volume = pow(box_size / (double)N, 3);
+  t.MT = (p->unit_mass_g / p->unit_time_s);
+  t.L3 = pow(p->unit_length_cm, -3.);
+  for (int c = 0; c < s->nsteps; ++c) {
+    t.dt_dzpp[c] = s->dt_dzpp[c];
+    t.dzpp[c] = s->dzpp[c];
+    t.zpp1[c] = 1 + s->zpp[c];
+    t.zpp_pow[c] = s->zpp_pow[c];
+    t.sum_lyn[c] = s->sum_lyn[c];
+    for (int x = 0; x < kNXHII; ++x) {
+      t.fheat[x][c] = s->freq_heat[x][c];
+      t.fion[x][c] = s->freq_ion[x][c];
+      t.flya[x][c] = s->freq_lya[x][c];
+    }
+  }
+  for (int x = 0; x < kNXHII; ++x) t.XH[x] = d->xi_XHII[x];
+  const float zp = (float)s->zp;
+  t.zp_f = zp;
+  t.dzp = s->dzp;
+  t.dt_dzp = s->dt_dzp;
+  t.const_zp_prefactor = s->const_zp_prefactor;
+  t.growth = s->growth_factor_zp;
+  t.dgrowth = s->dgrowth_factor_dzp;
+  t.NBP3 = d->N_b0 * pow(1 + zp, 3);
+  t.P2 = pow(1 + zp, 2);
+  t.dadia0 = 3 / (1.0 + zp);
+  t.Trad = 2.728 * (1.0 + zp);
+  t.T21_Trad = 0.0628 / t.Trad;
+  t.P3 = pow(1.0 + zp, 3.0);
+  const double H = mrx_ts_hubble(d, zp);
+  t.TGP = 1.342881e-7 / H * d->No * pow(1 + (double)zp, 3);
+  t.XA = 1.66e11 / (1.0 + zp);
+  t.HH = H / (3.2407789e-18 * p->hubble_h);
+  t.T4 = pow(t.Trad, 4.0);
+  t.one_p_z = 1.0 + zp;
+  t.Tk_floor = (float)(2.728 * (1 + s->zp));
+  t.No = d->No;
+  t.N_b0 = d->N_b0;
+  t.f_H = d->f_H;
+  t.f_He = d->f_He;
+  t.hubble_h = p->hubble_h;
+  fill_kappa(t.kHH, d->kappa_HH);
+  fill_kappa(t.keH, d->kappa_eH);
+  fill_kappa(t.kpH, d->kappa_pH);
+}
+
+std::string ts_key(const mrx_ts_params* p)
+{
+  char b[1400];
+  snprintf(b, sizeof b, "%s|%.17g|%.17g|%.17g|%.17g|%.17g|%.17g|%.17g|%.17g|%.17g", p->tables_dir, p->hubble_h,
+           p->omega_m, p->omega_lambda, p->omega_r, p->omega_k, p->w_lambda, p->baryon_frac, p->y_he,
+           p->nion_phot_per_bary);
+  return std::string(b) + "|" + std::to_string(p->reion_efficiency);
+}
+
+double ts_RtoM(const mrx_ts_params* p, double R)
+{
+  if (p->rtom_filter_type == 0) return (4.0 / 3.0) * M_PI * pow(R, 3) * (p->omega_m * p->rho_crit);
+  return pow(2 * M_PI, 1.5) * p->omega_m * p->rho_crit * pow(R, 3);
+}
+
+struct TsRun {
+  template <int N>
+  struct At {
+    // grids are device pointers here: deltax, stars, sfr, hist (slots), xe_prev, Tk, TS, xe
+    static int run(mrx_ctx* ctx, const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
+                   const float* hist, float* xe_prev, float* Tk, float* TS, mrx_ts_globals* g)
+    {
+      using P = Pipe<N>;
+      mrx_ts_data* d = ctx->tsd;
+      mrx_ts_snap* s = ctx->tss;
+      const long ncell = (long)N * N * N;
+      const size_t np = npad(N);
+      const double h = p->hubble_h;
+      const double box_size = p->box_size / h;
+      const double pixel_volume = pow(box_size / (double)N, 3);
+      const int nbc = (int)((ncell + kNT - 1) / kNT);
+      MRX_TRY(ensure_partials(ctx, (size_t)nbc * 7 + 2 * 1024 + 16));
+      if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
+      if (!ctx->ts_dev) HIP_TRY(hipMalloc(&ctx->ts_dev, sizeof(TsDev)));
+      double collapse_fraction = 0.0, x_e_ave = 0.0;
+      auto fcoll = [&](double R, int with_xe) -> int {
+        FcollArgs fa;
+        fa.M_h2 = ts_RtoM(p, R) * h * h;
+        fa.R3 = pow(R, 3.0);
+        fa.R3_over_V = 0.0;
+        fa.pixel_volume = pixel_volume;
+        fa.N = N;
+        fa.with_xe = with_xe;
+        {
+          PROF("ts_fcoll");
+          k_ts_fcoll<<<1024, kNT, 0, ctx->stream>>>(stars, deltax, xe_prev, ctx->partials + (size_t)nbc * 7, fa);
+        }
+        MRX_TRY(check_launch(ctx, "k_ts_fcoll"));
+        k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials + (size_t)nbc * 7, 1024, 2, ctx->sums);
+        MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+        double hsum[2];
+        HIP_TRY(hipMemcpyAsync(hsum, ctx->sums, sizeof hsum, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        collapse_fraction = hsum[0] / pow((double)N, 3);
+        x_e_ave = hsum[1] / pow((double)N, 3);
+        return MRX_OK;
+      };
+      if (s->homogeneous) {
+        const float zp = (float)s->zp;
+        const float xe0 = (float)mrx_ts_xion_RECFAST(d, zp);
+        const float T0 = (float)mrx_ts_T_RECFAST(d, zp);
+        fill_ts_dev(*ctx->ts_host, d, s, p);
+        HIP_TRY(hipMemcpyAsync(ctx->ts_dev, ctx->ts_host, sizeof(TsDev), hipMemcpyHostToDevice, ctx->stream));
+        {
+          PROF("ts_homogeneous");
+          k_ts_homogeneous<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, deltax, xe_prev, Tk, TS, N, xe0, T0,
+                                                         ctx->partials);
+        }
+        MRX_TRY(check_launch(ctx, "k_ts_homogeneous"));
+        MRX_TRY(fcoll(kLFactor * box_size / (float)N, 0));
+        x_e_ave = 0.0;
+      } else {
+        MRX_TRY(ensure_buffers(ctx, N, 1));
+        MRX_TRY(ensure_buf(ctx, ctx->ts_work, ctx->ts_work_cap, np));
+        MRX_TRY(ensure_buf(ctx, ctx->ts_shells, ctx->ts_shells_cap, (size_t)s->nsteps * ncell));
+        HIP_TRY(hipMemcpyAsync(ctx->ts_work, sfr, sizeof(float) * np, hipMemcpyDeviceToDevice, ctx->stream));
+        const long half = (long)N * N * (N / 2 + 1);
+        for (int ct = 0; ct < s->nsteps; ++ct) {
+          for (int k = 0; k < s->n_ops[ct]; ++k) {
+            const mrx_sfr_op& o = s->ops[ct][k];
+            const long n = (o.kind <= 1) ? half : ncell;
+            const float* hs = (o.kind >= 2) ? hist + (size_t)o.slot * np : nullptr;
+            PROF("ts_sfr_op");
+            k_sfr_op<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(ctx->ts_work, hs, o.kind, o.weight,
+                                                                          (float)o.weight, N, half);
+            MRX_TRY(check_launch(ctx, "k_sfr_op"));
+          }
+          // the working grid only changes when operations were applied: the
+          // forward transform of an unchanged grid is reused
+          if (ct == 0 || s->n_ops[ct] > 0) MRX_TRY(P::forward(ctx, ctx->ts_work, 0, 1, true, ctx->U, ctx->V, true));
+          const WinArgs w = win_args(p->box_size, (float)s->R[ct], p->heating_filter_type, ct > 0 ? 1 : 0);
+          MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
+          MRX_TRY(P::c2r_z(ctx, ctx->T, ctx->ts_shells + (size_t)ct * ncell, 1, 3, N));
+          if (ct == 0) MRX_TRY(fcoll(s->R[0], 1));
+        }
+        s->collapse_fraction = collapse_fraction;
+        s->x_e_ave = x_e_ave;
+        const int rc = mrx_ts_build_tables(s, d, p);
+        if (rc != MRX_OK) return set_err(ctx, rc, "spin-temperature tables failed");
+        fill_ts_dev(*ctx->ts_host, d, s, p);
+        HIP_TRY(hipMemcpyAsync(ctx->ts_dev, ctx->ts_host, sizeof(TsDev), hipMemcpyHostToDevice, ctx->stream));
+        {
+          PROF("ts_cell");
+          k_ts_cell<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, ctx->ts_shells, deltax, xe_prev, Tk, TS, N,
+                                                  ctx->partials);
+        }
+        MRX_TRY(check_launch(ctx, "k_ts_cell"));
+      }
+      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nbc, 7, ctx->sums);
+      MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+      double hs[7];
+      HIP_TRY(hipMemcpyAsync(hs, ctx->sums, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      const double n3 = pow((double)N, 3);
+      if (g) {
+        g->volume_ave_J_alpha = hs[0] / n3;
+        g->volume_ave_xalpha = hs[1] / n3;
+        g->volume_ave_Xheat = hs[2] / n3;
+        g->volume_ave_Xion = hs[3] / n3;
+        g->volume_ave_TS = hs[4] / n3;
+        g->volume_ave_TK = hs[5] / n3;
+        g->volume_ave_xe = hs[6] / n3;
+        if (p->reproduce_quirks) {
+          // ComputeTs.c:957 reads x_e_box_prev at the stale index of the previous loop: the last cell
+          float last = 0.f;
+          const size_t ip_last = ((size_t)N * N - 1) * (N + 2) + (N - 1);
+          HIP_TRY(hipMemcpy(&last, xe_prev + ip_last, sizeof(float), hipMemcpyDeviceToHost));
+          g->volume_ave_xe = (double)last * n3 / n3;
+        }
+        g->collapse_fraction = collapse_fraction;
+        g->x_e_ave = x_e_ave;
+        g->no_light = s->homogeneous ? 0 : s->no_light;
+        g->homogeneous = s->homogeneous;
+      }
+      return MRX_OK;
     }
   };
 };
@@ -500,6 +800,17 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->partials);
   dfree(ctx->sums);
   dfree(ctx->stage);
+  dfree(ctx->ts_dev);
+  dfree(ctx->ts_work);
+  dfree(ctx->ts_shells);
+  delete ctx->tsd;
+  delete ctx->tss;
+  delete ctx->ts_host;
+  for (auto& r : ctx->prof_recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return MRX_OK;
@@ -674,6 +985,66 @@ int mrx_filtered_field(mrx_ctx* ctx, int N, double box_size, int filter_type, co
   return MRX_OK;
 }
 
+
+int mrx_compute_ts(mrx_ctx* ctx, const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
+                   const float* sfr_histories, float* x_e_box, float* x_e_box_prev, float* Tk_box, float* TS_box,
+                   mrx_ts_globals* out)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!p) return set_err(ctx, MRX_EINVAL, "params is NULL");
+  if (!supported_n(p->grid_dim)) return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not supported", p->grid_dim);
+  if (p->num_filter_steps < 1 || p->num_filter_steps > kTsMaxSteps)
+    return set_err(ctx, MRX_EUNSUPPORTED, "TsNumFilterSteps %d outside 1..%d", p->num_filter_steps, kTsMaxSteps);
+  if (p->heating_filter_type < 0 || p->heating_filter_type > 2)
+    return set_err(ctx, MRX_EFILTER, "TsHeatingFilterType %d is undefined", p->heating_filter_type);
+  if (!deltax || !stars || !sfr || !x_e_box || !x_e_box_prev || !Tk_box || !TS_box || !p->redshifts ||
+      !p->stored_fcoll || !p->tables_dir || p->snapshot < 0)
+    return set_err(ctx, MRX_EINVAL, "a required pointer is NULL");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const std::string key = ts_key(p);
+  if (!ctx->tsd || ctx->ts_key != key) {
+    if (!ctx->tsd) ctx->tsd = new mrx_ts_data();
+    if (mrx_ts_data_init(ctx->tsd, p) != MRX_OK) {
+      ctx->ts_key.clear();
+      return set_err(ctx, MRX_EINVAL, "cannot read the 21cmFAST tables under %s", p->tables_dir);
+    }
+    ctx->ts_key = key;
+  }
+  if (!ctx->tss) ctx->tss = new mrx_ts_snap();
+  if (!ctx->ts_host) ctx->ts_host = new TsDev();
+  if (mrx_ts_plan(ctx->tss, ctx->tsd, p) != MRX_OK) return set_err(ctx, MRX_EINVAL, "bad spin-temperature plan");
+  const mrx_ts_snap* s = ctx->tss;
+  if (!s->homogeneous && s->max_slot >= 0 && (!sfr_histories || s->max_slot >= p->n_history))
+    return set_err(ctx, MRX_EINVAL, "the shells need SFR history slot %d, %d held", s->max_slot, p->n_history);
+  const int N = p->grid_dim;
+  const size_t np = npad(N), nr = nreal(N);
+  if (p->pointers_on_device) {
+    MRX_TRY(dispatch<TsRun>(N, ctx, p, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box, TS_box, out));
+    HIP_TRY(hipMemcpyAsync(x_e_box, x_e_box_prev, sizeof(float) * np, hipMemcpyDeviceToDevice, ctx->stream));
+    return finish(ctx);
+  }
+  const int nslot = s->homogeneous ? 0 : (s->max_slot + 1);
+  MRX_TRY(ensure_stage(ctx, (4 + (size_t)nslot) * np + 2 * nr));
+  float* st = ctx->stage;
+  float *d_dx = st, *d_st = st + np, *d_sfr = st + 2 * np, *d_xe = st + 3 * np;
+  float *d_Tk = st + 4 * np, *d_TS = st + 4 * np + nr, *d_hist = st + 4 * np + 2 * nr;
+  HIP_TRY(hipMemcpyAsync(d_dx, deltax, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(d_st, stars, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(d_sfr, sfr, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(d_xe, x_e_box_prev, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(d_Tk, Tk_box, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+  if (nslot > 0)
+    HIP_TRY(hipMemcpyAsync(d_hist, sfr_histories, sizeof(float) * np * nslot, hipMemcpyHostToDevice, ctx->stream));
+  MRX_TRY(dispatch<TsRun>(N, ctx, p, (const float*)d_dx, (const float*)d_st, (const float*)d_sfr,
+                          (const float*)d_hist, d_xe, d_Tk, d_TS, out));
+  HIP_TRY(hipMemcpyAsync(x_e_box_prev, d_xe, sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(x_e_box, d_xe, sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(Tk_box, d_Tk, sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(TS_box, d_TS, sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
 int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, const float* deltax,
                         const float* TS_box, float* delta_T, double* ave_Tb)
 {
@@ -712,7 +1083,10 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
   const int nb = 2048;
   MRX_TRY(ensure_partials(ctx, nb + 8));
   if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
+  {
+  PROF("brightness_temp");
   k_brightness_temp<<<nb, kNT, 0, ctx->stream>>>(dxH, ddx, dTS, ddT, ctx->partials, (long)nr, a);
+  }
   MRX_TRY(check_launch(ctx, "k_brightness_temp"));
   k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nb, 1, ctx->sums);
   MRX_TRY(check_launch(ctx, "k_reduce_partials"));
@@ -725,6 +1099,52 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
   return MRX_OK;
 }
 
+int mrx_profile(mrx_ctx* ctx, int enable)
+{
+  if (!ctx) return MRX_EINVAL;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->prof_recs) {
+    ctx->event_pool.push_back(r.a);
+    ctx->event_pool.push_back(r.b);
+  }
+  ctx->prof_recs.clear();
+  ctx->prof = enable != 0;
+  return MRX_OK;
+}
+
+int mrx_profile_read(mrx_ctx* ctx, int max_kinds, char* names, int name_len, double* ms_total, long* counts)
+{
+  if (!ctx || max_kinds < 0 || (max_kinds > 0 && (!names || !ms_total || !counts || name_len < 2)))
+    return MRX_EINVAL;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  std::vector<std::string> kinds;
+  std::vector<double> tot;
+  std::vector<long> cnt;
+  for (auto& r : ctx->prof_recs) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+    size_t k = 0;
+    while (k < kinds.size() && kinds[k] != r.name) ++k;
+    if (k == kinds.size()) {
+      kinds.push_back(r.name);
+      tot.push_back(0.0);
+      cnt.push_back(0);
+    }
+    tot[k] += ms;
+    cnt[k] += 1;
+    ctx->event_pool.push_back(r.a);
+    ctx->event_pool.push_back(r.b);
+  }
+  ctx->prof_recs.clear();
+  const int n = (int)kinds.size();
+  for (int k = 0; k < n && k < max_kinds; ++k) {
+    snprintf(names + (size_t)k * name_len, (size_t)name_len, "%s", kinds[k].c_str());
+    ms_total[k] = tot[k];
+    counts[k] = cnt[k];
+  }
+  return n;
+}
+
 // Size of each ABI struct, so bindings can check their mirrors.
 size_t mrx_abi_sizeof(int which)
 {
@@ -732,6 +1152,8 @@ size_t mrx_abi_sizeof(int which)
     case 0: return sizeof(mrx_hii_params);
     case 1: return sizeof(mrx_hii_globals);
     case 2: return sizeof(mrx_tb_params);
+    case 3: return sizeof(mrx_ts_params);
+    case 4: return sizeof(mrx_ts_globals);
     default: return 0;
   }
 }

@@ -105,9 +105,12 @@ __device__ __forceinline__ float window(int nx, int ny, int nz, int N, const Win
       kR = (float)((double)kR * 0.413566994);
       return (kR > 1.0f) ? 0.0f : 1.0f;
     default: {
+      // powf((float)M_E, y) = exp(y ln((float)M_E)), evaluated in double and
+      // rounded once: the correctly rounded value libm's powf returns
       kR = (float)((double)kR * 0.643);
-      const float e = (float)2.718281828459045;
-      return powf(e, (float)((double)(-kR * kR) / 2.0));
+      const float y = (float)((double)(-kR * kR) / 2.0);
+      const double ln_ef = 0.99999996963214000;  // log((double)(float)M_E)
+      return (float)exp((double)y * ln_ef);
     }
   }
 }
@@ -198,6 +201,8 @@ __device__ __forceinline__ float clamp_kind(float v, int kind)
       v = fmaxf(v, 0.0f);
       v = (v < kAbsTol) ? 0.0f : v;
       return fminf(v, 0.999f);
+    case 3:  // spin-temperature shells: max(v, 0) only (ComputeTs.c:417, :488)
+      return fmaxf(v, 0.0f);
     default:
       return v;
   }
@@ -207,7 +212,7 @@ __device__ __forceinline__ float clamp_kind(float v, int kind)
 template <int N>
 __global__ __launch_bounds__(kNT) void k_c2r_z(const float2* __restrict__ in, float* __restrict__ out,
                                               const float2* __restrict__ twN, long in_fstride, long out_fstride,
-                                              int clamp)
+                                              int clamp, int pitch)
 {
   using C = Cfg<N>;
   constexpr int RZ = C::RZ, ZE = C::ZE;
@@ -219,9 +224,9 @@ __global__ __launch_bounds__(kNT) void k_c2r_z(const float2* __restrict__ in, fl
   c2r_rows_lds<N>(s, tw, in + blockIdx.y * in_fstride, row0, tid);
   const float* sf = reinterpret_cast<const float*>(s);
   float* o = out + blockIdx.y * out_fstride;
-  for (int e = tid; e < RZ * (N + 2); e += kNT) {
-    const int r = e / (N + 2), z = e % (N + 2);
-    o[(row0 + r) * (N + 2) + z] = (z < N) ? clamp_kind(sf[r * N + z], clamp) : 0.0f;
+  for (int e = tid; e < RZ * pitch; e += kNT) {
+    const int r = e / pitch, z = e % pitch;
+    o[(row0 + r) * pitch + z] = (z < N) ? clamp_kind(sf[r * N + z], clamp) : 0.0f;
   }
 }
 
@@ -250,7 +255,12 @@ __global__ __launch_bounds__(Cfg<N>::COL_NT) void k_colpass(float2* __restrict__
   for (int e = tid; e < N * kTW; e += NT) {
     const int n = e / kTW, c = e % kTW;
     float2 v = s[e];
-    if (div != 0.0) v = make_float2((float)((double)v.x / div), (float)((double)v.y / div));
+    if (div > 0.0) {  // complex float / double, in double (find_HII_bubbles.c:168-182)
+      v = make_float2((float)((double)v.x / div), (float)((double)v.y / div));
+    } else if (div < 0.0) {  // complex float / float (ComputeTs.c:383-384)
+      const float fd = (float)(-div);
+      v = make_float2(v.x / fd, v.y / fd);
+    }
     g[base + n * rstride + c] = v;
   }
 }

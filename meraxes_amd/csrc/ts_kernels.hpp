@@ -1,0 +1,375 @@
+// ts_kernels.hpp - HIP kernels of the spin-temperature step (_ComputeTs).
+//
+// The shell loop reuses the bubble finder's transform kernels: the working SFR
+// grid is transformed, windowed at each shell radius and brought back to real
+// space, and the clamped shell (float) is stored shell-major, [shell][cell],
+// so the per-cell kernel reads one coalesced row per shell.  The reference
+// keeps the shells as double (ComputeTs.c:420-433); the double it stores is a
+// fixed product of the float shell value, so the per-cell kernel rebuilds the
+// identical double from the float and the store costs 4 instead of 8 bytes.
+//
+// The per-cell kernel is ComputeTs.c:743-900 with evolveInt
+// (XRayHeatingFunctions.c:1241-1435) and get_Ts (:1513-1558) in FP64; every
+// cell-independent sub-expression that the reference evaluates first (a
+// left-to-right prefix) is evaluated once on the host and passed in, so the
+// device repeats the reference's own operation order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "reion_kernels.hpp"
+
+namespace mrx {
+
+constexpr int kTsMaxSteps = 64;
+constexpr int kNXHII = 14;
+
+struct KappaSpline {
+  int n;
+  double x[27], y[27], b[27], c[27], d[27];
+};
+
+// Everything the per-cell kernel needs, uploaded once per snapshot.
+struct TsDev {
+  int nsteps;
+  int no_light;
+  // shell value -> SMOOTHED_SFR double: ((f / V) * MT) * L3 / MSUN
+  double pixel_volume, MT, L3;
+  // per-shell factors
+  double dt_dzpp[kTsMaxSteps], dzpp[kTsMaxSteps], zpp1[kTsMaxSteps], zpp_pow[kTsMaxSteps], sum_lyn[kTsMaxSteps];
+  double fheat[kNXHII][kTsMaxSteps], fion[kNXHII][kTsMaxSteps], flya[kNXHII][kTsMaxSteps];
+  float XH[kNXHII];
+  // snapshot constants
+  double dzp, dt_dzp, const_zp_prefactor, growth, dgrowth;
+  double NBP3;      // N_b0 * pow(1 + zp_f, 3), 1 + zp_f in float
+  double P2;        // pow(1 + zp_f, 2), 1 + zp_f in float
+  double dadia0;    // 3 / (1.0 + zp_f)
+  double Trad;      // TCMB (1.0 + zp_f)
+  double T21_Trad;  // T21 / Trad
+  double P3;        // pow(1.0 + zp_f, 3.0)
+  double TGP;       // (1.342881e-7 / hubble((float)zp)) * No * pow(1 + z, 3)
+  double XA;        // 1.66e11 / (1.0 + zp_f)
+  double HH;        // hubble((float)zp) / (HUBBLE h)
+  double T4;        // pow(Trad, 4.0)
+  double one_p_z;   // 1.0 + zp_f
+  double Tk_floor;  // (float)(TCMB (1 + zp)) with zp double
+  double No, N_b0, f_H, f_He, hubble_h;
+  float zp_f;
+  KappaSpline kHH, keH, kpH;
+};
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ double spline_eval(const KappaSpline& s, double xv)
+{
+  int lo = 0, hi = s.n - 1;
+  if (xv <= s.x[0]) {
+    lo = 0;
+  } else if (xv >= s.x[hi]) {
+    lo = hi - 1;
+  } else {
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (s.x[mid] > xv)
+        hi = mid;
+      else
+        lo = mid;
+    }
+  }
+  const double dx = xv - s.x[lo];
+  return s.y[lo] + dx * (s.b[lo] + dx * (s.c[lo] + dx * s.d[lo]));
+}
+
+// kappa_10 (H-H) with its below-table clamp and T^0.381 extrapolation (XRayHeatingFunctions.c:1684-1693)
+__device__ __forceinline__ double kappa_HH(const KappaSpline& s, double TK)
+{
+  double ans;
+  if (log(TK) < s.x[0])
+    ans = s.y[0];
+  else if (log(TK) > s.x[s.n - 1])
+    ans = log(exp(s.y[s.n - 1]) * pow(TK / exp(s.x[s.n - 1]), 0.381));
+  else
+    ans = spline_eval(s, log(TK));
+  return exp(ans);
+}
+
+// kappa_10_elec / kappa_10_pH with linear-in-log extrapolation (:1754-1766, :1827-1839)
+__device__ __forceinline__ double kappa_file(const KappaSpline& s, double T)
+{
+  const int n = s.n;
+  double ans;
+  T = log(T);
+  if (T < s.x[0])
+    ans = s.y[0];
+  else if (T > s.x[n - 1])
+    ans = s.y[n - 1] + ((s.y[n - 1] - s.y[n - 2]) / (s.x[n - 1] - s.x[n - 2]) * (T - s.x[n - 1]));
+  else
+    ans = spline_eval(s, T);
+  return exp(ans);
+}
+
+// get_Ts (XRayHeatingFunctions.c:1513-1558) at this snapshot's z; float in, float out
+__device__ __forceinline__ float dev_get_Ts(const TsDev& t, float delta, float TK_f, float xe_f, float Jalpha,
+                                           float* xa_out)
+{
+  const double TK = TK_f, xe = xe_f, dl = delta;
+  const double Trad = t.Trad;
+  // xcoll = xcoll_HI + xcoll_elec + xcoll_prot (:1560-1600)
+  const double nH = (1.0 - xe) * t.No * t.P3 * (1.0 + dl);
+  const double xHI = t.T21_Trad * nH * kappa_HH(t.kHH, TK) / 2.85e-15;
+  const double ne = xe * t.N_b0 * t.P3 * (1.0 + dl);
+  const double xel = t.T21_Trad * ne * kappa_file(t.keH, TK) / 2.85e-15;
+  const double np = xe * t.No * t.P3 * (1.0 + dl);
+  const double xpr = t.T21_Trad * np * kappa_file(t.kpH, TK) / 2.85e-15;
+  const double xc = xHI + xel + xpr;
+  double TS;
+  if (Jalpha > 1.0e-20) {
+    double TSold = 0.0, xa = 0.0;
+    TS = Trad;
+    const double tgp = t.TGP * (1.0 + dl) * (1.0 - xe);
+    while (fabs(TS - TSold) / TS > 1.0e-3) {
+      TSold = TS;
+      const double xi = pow(1.0e-7 * tgp / TK / TK, 1.0 / 3.0);
+      double S = 1.0 - 0.0631789 / TK + 0.115995 / TK / TK - 0.401403 / TS / TK;
+      S += 0.336463 / TS / TK / TK;
+      S /= 1.0 + 2.98394 * xi + 1.53583 * xi * xi + 3.85289 * xi * xi * xi;
+      xa = t.XA * S * (double)Jalpha;
+      const double ce = 1.0 / TK + 0.405535 / TK * (1.0 / TS - 1.0 / TK);
+      const double Tceff = 1.0 / ce;
+      const double TSinv = (1.0 / Trad + xa / Tceff + xc / TK) / (1.0 + xa + xc);
+      TS = 1.0 / TSinv;
+    }
+    *xa_out = (float)xa;
+  } else {
+    const double TSinv = (1.0 / Trad + xc / TK) / (1.0 + xc);
+    TS = 1.0 / TSinv;
+    *xa_out = 0.0f;
+  }
+  if (TS < 0.) TS = fabs(TS);
+  return (float)TS;
+}
+
+__device__ __forceinline__ double dev_alpha_A(double T)
+{
+  const double logT = log(T / (double)1.1604505e4);
+  return exp(-28.6130338 - 0.72411256 * logT - 2.02604473e-2 * pow(logT, 2) - 2.38086188e-3 * pow(logT, 3) -
+             3.21260521e-4 * pow(logT, 4) - 1.42150291e-5 * pow(logT, 5) + 4.98910892e-6 * pow(logT, 6) +
+             5.75561414e-7 * pow(logT, 7) - 1.85676704e-8 * pow(logT, 8) - 3.07113524e-9 * pow(logT, 9));
+}
+
+// Per-cell spin-temperature update (ComputeTs.c:743-900); 7 block partial sums:
+// J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e.
+__global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, const float* __restrict__ shells,
+                                                const float* __restrict__ deltax, float* __restrict__ xe_prev,
+                                                float* __restrict__ Tk, float* __restrict__ TS, int N,
+                                                double* __restrict__ partials)
+{
+  __shared__ TsDev t;
+  __shared__ double red[kNT];
+  {
+    const int words = (int)(sizeof(TsDev) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(tg);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&t);
+    for (int i = threadIdx.x; i < words; i += kNT) dst[i] = src[i];
+  }
+  __syncthreads();
+  const long ncell = (long)N * N * N;
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  double sJ = 0, sxa = 0, sXh = 0, sXi = 0, sTS = 0, sTK = 0, sxe = 0;
+  if (i < ncell) {
+    const long r = i / N;
+    const int z = (int)(i % N);
+    const long ip = r * (N + 2) + z;
+    const float dl = deltax[ip];
+    const float xe0 = xe_prev[ip];
+    const double x_e = xe0, T = Tk[i];
+    // table interpolation at the cell's x_e (:759-790); the bracket is the same for every shell
+    double xHII_call = xe0;
+    if (xHII_call > t.XH[kNXHII - 1] * 0.999)
+      xHII_call = t.XH[kNXHII - 1] * 0.999;
+    else if (xHII_call < t.XH[0])
+      xHII_call = 1.001 * t.XH[0];
+    int lo = kNXHII - 1;
+    const float xc_f = (float)xHII_call;
+    while (xc_f < t.XH[lo]) lo--;
+    const int hi = lo + 1;
+    const double dxh = (double)(t.XH[hi] - t.XH[lo]);
+    const double fr = xHII_call - t.XH[lo];
+    const double n_b = t.NBP3 * (1 + dl);
+    double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
+    if (!t.no_light) {
+      for (int ct = 0; ct < t.nsteps; ++ct) {
+        const float sv = shells[(long)ct * ncell + i];
+        const double SFR = ((double)sv / t.pixel_volume) * t.MT * t.L3 / 1.989e33;
+        double fh = (t.fheat[hi][ct] - t.fheat[lo][ct]) / dxh;
+        fh *= fr;
+        fh += t.fheat[lo][ct];
+        double fi = (t.fion[hi][ct] - t.fion[lo][ct]) / dxh;
+        fi *= fr;
+        fi += t.fion[lo][ct];
+        double fl = (t.flya[hi][ct] - t.flya[lo][ct]) / dxh;
+        fl *= fr;
+        fl += t.flya[lo][ct];
+        const double dt_dzpp = t.dt_dzpp[ct], dzpp = t.dzpp[ct];
+        const double zpp_integrand = SFR * t.zpp_pow[ct];
+        dxheat += dt_dzpp * dzpp * zpp_integrand * fh;
+        dxion += dt_dzpp * dzpp * zpp_integrand * fi;
+        dxlya += dt_dzpp * dzpp * zpp_integrand * fl;
+        dstarlya += SFR * t.P2 * t.zpp1[ct] * t.sum_lyn[ct] * dt_dzpp * dzpp;
+      }
+      dxheat *= t.const_zp_prefactor;
+      dxion *= t.const_zp_prefactor;
+      dxlya *= t.const_zp_prefactor * n_b;
+      dstarlya *= (2.9979e10 / (4. * M_PI)) / (1.6726e-24 / 1.989e33);
+    }
+    const double sink = dev_alpha_A(T) * 2.0 * x_e * x_e * t.f_H * n_b;
+    const double dxe_dzp = t.dt_dzp * (dxion - sink);
+    double dadia = t.dadia0;
+    if (fabs(dl) > 1e-7) dadia += t.dgrowth / (1.0 / dl + t.growth);
+    dadia *= (2.0 / 3.0) * T;
+    const double dspec = -dxe_dzp * T / (1 + x_e);
+    double dcomp = (-1.51e-4) * (x_e / (1.0 + x_e + t.f_He)) / t.HH / t.hubble_h * t.T4 / t.one_p_z;
+    dcomp *= t.Trad - T;
+    const double dxheat_dzp = dxheat * t.dt_dzp * 2.0 / 3.0 / 1.3806e-16 / (1.0 + x_e);
+    const double der1 = dxheat_dzp + dcomp + dspec + dadia;
+    const double der2 = dxlya + dstarlya;
+    const double der4 = t.dt_dzp * dxion;
+    float xn = (float)((double)xe0 + dxe_dzp * t.dzp);
+    if (xn > 1)
+      xn = (float)(1 - 1e-7);
+    else if (xn < 0)
+      xn = 0;
+    float Tn = (float)T;
+    if (Tn < (float)5e4) Tn = (float)((double)Tn + der1 * t.dzp);
+    if (Tn < 0) Tn = (float)t.Tk_floor;
+    float xa;
+    const float ts = dev_get_Ts(t, dl, Tn, xn, (float)der2, &xa);
+    xe_prev[ip] = xn;
+    Tk[i] = Tn;
+    TS[i] = ts;
+    sJ = der2;
+    sxa = xa;
+    sXh = dxheat_dzp;
+    sXi = der4;
+    sTS = ts;
+    sTK = Tn;
+    sxe = xn;
+  }
+  double* p = partials + (long)blockIdx.x * 7;
+  const double v0 = block_sum<kNT>(sJ, red), v1 = block_sum<kNT>(sxa, red), v2 = block_sum<kNT>(sXh, red);
+  const double v3 = block_sum<kNT>(sXi, red), v4 = block_sum<kNT>(sTS, red), v5 = block_sum<kNT>(sTK, red);
+  const double v6 = block_sum<kNT>(sxe, red);
+  if (threadIdx.x == 0) {
+    p[0] = v0;
+    p[1] = v1;
+    p[2] = v2;
+    p[3] = v3;
+    p[4] = v4;
+    p[5] = v5;
+    p[6] = v6;
+  }
+}
+
+// Homogeneous branch (ComputeTs.c:189-217): RECFAST x_e and T_K everywhere, T_S
+// from collisions only; block sums of T_S, T_K, x_e (slots 4..6).
+__global__ __launch_bounds__(kNT) void k_ts_homogeneous(const TsDev* __restrict__ tg, const float* __restrict__ deltax,
+                                                       float* __restrict__ xe_prev, float* __restrict__ Tk,
+                                                       float* __restrict__ TS, int N, float xe0, float T0,
+                                                       double* __restrict__ partials)
+{
+  __shared__ TsDev t;
+  __shared__ double red[kNT];
+  {
+    const int words = (int)(sizeof(TsDev) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(tg);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&t);
+    for (int i = threadIdx.x; i < words; i += kNT) dst[i] = src[i];
+  }
+  __syncthreads();
+  const long ncell = (long)N * N * N;
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  double sTS = 0, sTK = 0, sxe = 0;
+  if (i < ncell) {
+    const long r = i / N;
+    const int z = (int)(i % N);
+    const long ip = r * (N + 2) + z;
+    float xa;
+    const float ts = dev_get_Ts(t, deltax[ip], T0, xe0, 0.0f, &xa);
+    xe_prev[ip] = xe0;
+    Tk[i] = T0;
+    TS[i] = ts;
+    sTS = ts;
+    sTK = T0;
+    sxe = xe0;
+  }
+  double* p = partials + (long)blockIdx.x * 7;
+  const double v4 = block_sum<kNT>(sTS, red), v5 = block_sum<kNT>(sTK, red), v6 = block_sum<kNT>(sxe, red);
+  if (threadIdx.x == 0) {
+    p[0] = p[1] = p[2] = p[3] = 0.0;
+    p[4] = v4;
+    p[5] = v5;
+    p[6] = v6;
+  }
+}
+
+// Mean collapse fraction of the first shell and <x_e> (ComputeTs.c:413-468):
+// per cell min(stars / (RtoM(R) h^2 (1+delta)) * (4/3) pi R^3 / V, 1).
+struct FcollArgs {
+  double M_h2;         // RtoM(R) * h * h
+  double R3_over_V;    // factor pieces, applied in the reference's order
+  double R3;           // pow(R, 3.0)
+  double pixel_volume;
+  int N;
+  int with_xe;
+};
+
+__global__ __launch_bounds__(kNT) void k_ts_fcoll(const float* __restrict__ stars, const float* __restrict__ deltax,
+                                                 const float* __restrict__ xe_prev, double* __restrict__ partials,
+                                                 FcollArgs a)
+{
+  __shared__ double red[kNT];
+  const long ncell = (long)a.N * a.N * a.N;
+  double sf = 0.0, sx = 0.0;
+  const long stride = (long)gridDim.x * kNT;
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < ncell; i += stride) {
+    const long r = i / a.N;
+    const int z = (int)(i % a.N);
+    const long ip = r * (a.N + 2) + z;
+    const double dom = 1.0 + deltax[ip];
+    double c = stars[ip] / (a.M_h2 * dom) * (4.0 / 3.0) * M_PI * a.R3 / a.pixel_volume;
+    if (c > 1.0) c = 1.0;
+    sf += c;
+    if (a.with_xe) sx += xe_prev[ip];
+  }
+  const double v0 = block_sum<kNT>(sf, red), v1 = block_sum<kNT>(sx, red);
+  if (threadIdx.x == 0) {
+    partials[(long)blockIdx.x * 2] = v0;
+    partials[(long)blockIdx.x * 2 + 1] = v1;
+  }
+}
+
+// Working SFR grid operations (ComputeTs.c:296-376, reionization.c:1756-1784).
+// kind 0: zero the first `half` floats; 1: scale them by a double weight;
+// 2: load hist (read at the unpadded index) * float weight into the padded
+// cell; 3: add it.
+__global__ void k_sfr_op(float* __restrict__ W, const float* __restrict__ hist, int kind, double wd, float wf, int N,
+                         long half)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kind <= 1) {
+    if (i >= half) return;
+    W[i] = (kind == 0) ? 0.0f : (float)((double)W[i] * wd);
+    return;
+  }
+  const long ncell = (long)N * N * N;
+  if (i >= ncell) return;
+  const long r = i / N;
+  const int z = (int)(i % N);
+  const float v = hist[i] * wf;
+  const long ip = r * (N + 2) + z;
+  W[ip] = (kind == 2) ? v : W[ip] + v;
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace mrx

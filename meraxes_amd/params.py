@@ -7,7 +7,12 @@ input/params/defaults.par:196-254, and the unit derivations of set_units
 """
 import math
 
-from .abi import HiiParams, TbParams
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import HiiParams, TbParams, TsParams
 
 # src/meraxes.h:21-44
 GRAVITY = 6.672e-8
@@ -94,3 +99,57 @@ def radius_ladder(p):
         if last:
             return out
         R /= p.delta_r_factor
+
+
+TABLES_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "21cmfast_tables")
+
+
+def ts_params(grid_dim, box_size, snapshot, redshifts, stored_fcoll, n_history, efficiency=20.0,
+              max_heating_redshift=20.0, num_filter_steps=40, heating_filter_type=1, on_device=0,
+              reproduce_quirks=0):
+    """mrx_ts_params with the defaults.par X-ray knobs (defaults.par:48-55, :247-254).
+
+    ``redshifts`` and ``stored_fcoll`` are float64 numpy arrays the struct points
+    into; keep them alive while the struct is used.
+    """
+    u = units()
+    p = TsParams()
+    p.grid_dim = grid_dim
+    p.pointers_on_device = on_device
+    p.num_filter_steps = num_filter_steps
+    p.heating_filter_type = heating_filter_type
+    p.rtom_filter_type = 0
+    p.snapshot = snapshot
+    p.n_history = n_history
+    p.reproduce_quirks = reproduce_quirks
+    p.box_size = box_size
+    p.hubble_h = HUBBLE_H
+    p.omega_m = OMEGA_M
+    p.omega_lambda = OMEGA_L
+    p.omega_r = 0.0
+    p.omega_k = 0.0
+    p.w_lambda = -1.0
+    p.baryon_frac = BARYON_FRAC
+    p.y_he = Y_HE
+    p.l_xray_gal = 3.16e40
+    p.nu_xray_gal_threshold = 500.0
+    p.spec_index_xray_gal = 1.0
+    p.nu_xray_soft_cut = 2000.0
+    p.nu_xray_max = 10000.0
+    p.reion_efficiency = efficiency
+    p.nion_phot_per_bary = 4000.0
+    p.max_heating_redshift = max_heating_redshift
+    p.rho_crit = u["rho_crit"]
+    p.unit_length_cm = UNIT_LENGTH_CM
+    p.unit_mass_g = UNIT_MASS_G
+    p.unit_time_s = u["unit_time_s"]
+    assert redshifts.dtype == np.float64 and stored_fcoll.dtype == np.float64
+    p.redshifts = redshifts.ctypes.data_as(C.POINTER(C.c_double))
+    p.stored_fcoll = stored_fcoll.ctypes.data_as(C.POINTER(C.c_double))
+    p.tables_dir = TABLES_DIR.encode()
+    return p
+
+
+def snapshot_redshifts(n=30, z_hi=20.0, z_lo=6.0):
+    """The 30 snapshot redshifts of the full-snapshot configuration, z = 20 .. 6."""
+    return np.linspace(z_hi, z_lo, n).astype(np.float64)

@@ -7,6 +7,7 @@ Same names and argument meaning as the reference functions each call replaces:
 * ``fft_r2c_3d`` / ``fft_c2r_3d`` -> the r2c / c2r plans (reionization.c:685-736)
 * ``filtered_field``    -> one field of the smoothing chain (find_HII_bubbles.c:214-311)
 * ``brightness_temp``   -> ComputeBrightnessTemperatureBox (BrightnessTemperature.c:24-549)
+* ``compute_ts``        -> _ComputeTs (ComputeTs.c:78-1020)
 
 Grids are numpy arrays (host memory, staged by the library) or torch tensors
 on the GPU (used in place, on torch's current stream).  Layouts are the
@@ -19,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import Context, MrxError, load  # noqa: F401
-from .abi import HiiGlobals
+from .abi import HiiGlobals, TsGlobals, TsParams
 
 try:  # torch is optional for host-memory use
     import torch
@@ -168,3 +169,48 @@ def brightness_temp(ctx, params, xH, deltax, TS_box=None, out=None):
     ctx.check(ctx.lib.mrx_brightness_temp(ctx.ptr, C.byref(params), _ptr(xH), _ptr(deltax), _ptr(TS_box), _ptr(out),
                                           C.byref(ave)), "brightness_temp")
     return out, ave.value
+
+
+def profile(ctx, enable=True):
+    """Start (or stop) per-kernel event timing on the context's stream."""
+    ctx.check(ctx.lib.mrx_profile(ctx.ptr, 1 if enable else 0), "profile")
+
+
+def profile_read(ctx, max_kinds=64):
+    """{kernel name: (total ms, launches)} since the last read; clears the record."""
+    name_len = 64
+    names = C.create_string_buffer(max_kinds * name_len)
+    ms = np.zeros(max_kinds, np.float64)
+    cnt = np.zeros(max_kinds, np.int64)
+    n = ctx.lib.mrx_profile_read(ctx.ptr, max_kinds, names, name_len, ms.ctypes.data_as(C.c_void_p),
+                                 cnt.ctypes.data_as(C.c_void_p))
+    ctx.check(0 if n >= 0 else n, "profile_read")
+    out = {}
+    raw = names.raw
+    for k in range(min(n, max_kinds)):
+        nm = raw[k * name_len:(k + 1) * name_len].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[k]), int(cnt[k]))
+    return out
+
+
+def compute_ts(ctx, params, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box, x_e_box=None, TS_box=None):
+    """One spin-temperature step; x_e_box_prev and Tk_box are updated in place.
+
+    Returns the output grids and the volume averages.  ``sfr_histories`` holds
+    params.n_history padded grids, slot k being k snapshots back.
+    """
+    N = params.grid_dim
+    on_dev = _is_dev(deltax)
+    params.pointers_on_device = 1 if on_dev else 0
+    if x_e_box is None:
+        x_e_box = torch.empty_like(x_e_box_prev) if on_dev else np.empty_like(x_e_box_prev)
+    if TS_box is None:
+        TS_box = _empty_like_real(deltax, N)
+    g = TsGlobals()
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_compute_ts(ctx.ptr, C.byref(params), _ptr(deltax), _ptr(stars), _ptr(sfr),
+                                     _ptr(sfr_histories), _ptr(x_e_box), _ptr(x_e_box_prev), _ptr(Tk_box),
+                                     _ptr(TS_box), C.byref(g)), "compute_ts")
+    out = {k: getattr(g, k) for k, _ in TsGlobals._fields_}
+    out.update({"x_e_box": x_e_box, "x_e_box_prev": x_e_box_prev, "Tk_box": Tk_box, "TS_box": TS_box})
+    return out

@@ -65,6 +65,13 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
                          const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
                          float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out, float* max_ratio);
 
+/* _ComputeTs restated (ts.c).  Same pointer contract as mrx_compute_ts (host
+ * memory); shells_out, if not NULL, receives the num_filter_steps smoothed SFR
+ * shells (float, shell-major, unpadded) of the inhomogeneous branch. */
+int orc_compute_ts(const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
+                   const float* sfr_histories, float* x_e_box, float* x_e_box_prev, float* Tk_box, float* TS_box,
+                   mrx_ts_globals* out, float* shells_out);
+
 /* ComputeBrightnessTemperatureBox restated (no peculiar velocities). */
 int orc_brightness_temp(const mrx_tb_params* p, const float* xH, const float* deltax, const float* TS_box,
                         float* delta_T, double* ave_Tb);

@@ -35,6 +35,8 @@ def test_struct_mirrors_match_library():
     assert L.mrx_abi_sizeof(0) == C.sizeof(abi.HiiParams)
     assert L.mrx_abi_sizeof(1) == C.sizeof(abi.HiiGlobals)
     assert L.mrx_abi_sizeof(2) == C.sizeof(abi.TbParams)
+    assert L.mrx_abi_sizeof(3) == C.sizeof(abi.TsParams)
+    assert L.mrx_abi_sizeof(4) == C.sizeof(abi.TsGlobals)
 
 
 def test_radius_ladder_matches_oracle_and_closed_form():

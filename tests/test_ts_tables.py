@@ -1,0 +1,182 @@
+"""Pins the spin-temperature host tables (meraxes_amd/csrc/ts_tables.c) and the
+oracle's per-cell restatement against scipy and closed forms."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, interpolate, optimize
+
+from meraxes_amd import params as P
+from meraxes_amd import synth
+from oracle import oracle as O
+
+FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+
+
+@pytest.fixture(scope="module")
+def L():
+    lib = O.lib()
+    lib.orc_ts_data_new.restype = C.c_void_p
+    lib.orc_ts_data_new.argtypes = [C.POINTER(P.TsParams)]
+    lib.orc_ts_data_free.argtypes = [C.c_void_p]
+    lib.orc_ts_scalar.restype = C.c_double
+    lib.orc_ts_scalar.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    lib.orc_ts_No.restype = C.c_double
+    lib.orc_ts_No.argtypes = [C.c_void_p, C.c_int]
+    lib.orc_qag61.restype = C.c_double
+    lib.orc_qag61.argtypes = [FN, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    lib.orc_brent.restype = C.c_double
+    lib.orc_brent.argtypes = [FN, C.c_double, C.c_double, C.c_double]
+    lib.orc_spline.restype = C.c_double
+    lib.orc_spline.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double]
+    lib.orc_ts_plan_summary.argtypes = [C.POINTER(P.TsParams), C.c_void_p, C.c_void_p, C.c_void_p]
+    return lib
+
+
+@pytest.fixture(scope="module")
+def data(L):
+    z = P.snapshot_redshifts()
+    fc = np.zeros(len(z))
+    p = P.ts_params(32, 25.0, 3, z, fc, 8)
+    d = L.orc_ts_data_new(C.byref(p))
+    assert d, "tables must load"
+    yield d, p, z, fc
+    L.orc_ts_data_free(d)
+
+
+def test_natural_spline_matches_scipy(L):
+    rng = np.random.default_rng(0)
+    x = np.sort(rng.random(17)) * 10
+    y = np.sin(x) + rng.random(17)
+    cs = interpolate.CubicSpline(x, y, bc_type="natural")
+    for xv in np.linspace(x[0], x[-1], 97):
+        v = L.orc_spline(x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), 17, xv)
+        assert abs(v - cs(xv)) <= 1e-12 * (1 + abs(cs(xv)))
+
+
+def test_qag61_matches_quad(L):
+    f = FN(lambda x, _: math.exp(-x) * math.sin(3 * x) / (1 + x * x))
+    err = C.c_double()
+    got = L.orc_qag61(f, 0.0, 20.0, 1e-10, C.byref(err))
+    ref = integrate.quad(lambda x: math.exp(-x) * math.sin(3 * x) / (1 + x * x), 0, 20, epsabs=0, epsrel=1e-13)[0]
+    assert abs(got - ref) <= 1e-10 * abs(ref)
+    # a kinked integrand needs subdivision; the 1% tolerance the reference uses is met
+    g = FN(lambda x, _: abs(x - 0.3) ** 0.5)
+    got = L.orc_qag61(g, 0.0, 1.0, 0.01, C.byref(err))
+    ref = (2 / 3) * (0.3**1.5 + 0.7**1.5)
+    assert abs(got - ref) <= 0.01 * ref
+
+
+def test_brent_matches_brentq(L):
+    f = FN(lambda x, _: math.cos(x) - x)
+    got = L.orc_brent(f, 0.0, 1.0, 1e-12)
+    assert abs(got - optimize.brentq(lambda x: math.cos(x) - x, 0, 1, xtol=1e-15)) < 1e-11
+    got = L.orc_brent(f, 0.0, 1.0, 0.02)  # the reference's nu_tau_one tolerance
+    assert abs(got - 0.7390851332) <= 0.02 * 0.74
+
+
+def test_recfast_and_kappa_reproduce_their_tables(L, data):
+    d = data[0]
+    tab = np.loadtxt(P.TABLES_DIR + "/recfast_LCDM.dat")
+    for row in tab[::37]:
+        z, xe, _, T = row
+        assert abs(L.orc_ts_scalar(d, 0, z) - np.float32(T)) <= 1e-9 * abs(T) + 1e-12
+        assert abs(L.orc_ts_scalar(d, 1, z) - np.float32(xe)) <= 1e-9 * abs(xe) + 1e-15
+    # H-H rate at the Zygelman knots, and the T^0.381 extrapolation above 1e4 K
+    for T, k in [(1.0, 1.38e-13), (30.0, 3.67e-11), (1000.0, 3.31e-10), (10000.0, 7.87e-10)]:
+        assert L.orc_ts_scalar(d, 2, T) == pytest.approx(k, rel=1e-12)
+    assert L.orc_ts_scalar(d, 2, 4e4) == pytest.approx(7.87e-10 * 4**0.381, rel=1e-12)
+    assert L.orc_ts_scalar(d, 2, 0.5) == pytest.approx(1.38e-13, rel=1e-12)
+    eH = np.loadtxt(P.TABLES_DIR + "/kappa_eH_table.dat", max_rows=20)
+    for T, k in eH[::4]:
+        assert L.orc_ts_scalar(d, 3, np.float32(T)) == pytest.approx(np.float32(k), rel=1e-6)
+
+
+def test_cosmology_helpers(L, data):
+    d = data[0]
+    h, Om, Ol = P.HUBBLE_H, P.OMEGA_M, P.OMEGA_L
+    H0 = P.HUBBLE * h
+    for z in (6.0, 9.5, 20.0):
+        zf = float(np.float32(z))
+        assert L.orc_ts_scalar(d, 6, z) == pytest.approx(H0 * math.sqrt(Om * (1 + zf) ** 3 + Ol), rel=1e-12)
+        # dt/dz = -1 / ((1+z) H(z)) for flat LCDM
+        assert L.orc_ts_scalar(d, 5, z) == pytest.approx(-1 / ((1 + zf) * H0 * math.sqrt(Om * (1 + zf) ** 3 + Ol)),
+                                                        rel=1e-10)
+    # growth factor normalised to 1 today, within the Carroll-Press-Turner fit's accuracy of the exact integral
+    assert L.orc_ts_scalar(d, 7, 0.0) == pytest.approx(1.0, rel=1e-12)
+
+    def D_exact(a):
+        E = lambda a_: math.sqrt(Om / a_**3 + Ol)  # noqa: E731
+        return E(a) * integrate.quad(lambda a_: 1 / (a_ * E(a_)) ** 3, 0, a, epsabs=0, epsrel=1e-12)[0]
+
+    for z in (6.0, 10.0, 20.0):
+        ex = D_exact(1 / (1 + z)) / D_exact(1.0)
+        assert L.orc_ts_scalar(d, 7, z) == pytest.approx(ex, rel=0.02)
+    # case-A recombination at 1e4 K (Abel et al. 1997 fit), ~4.2e-13 cm^3/s
+    assert 3.5e-13 < L.orc_ts_scalar(d, 8, 1e4) < 5e-13
+    # hydrogen number density today: rho_crit Omega_b (1-Y)/m_p
+    rhoc = 3 * (P.HUBBLE * h) ** 2 / (8 * math.pi * P.GRAVITY)
+    assert L.orc_ts_No(d, 0) == pytest.approx(rhoc * Om * P.BARYON_FRAC * (1 - P.Y_HE) / 1.6726e-24, rel=1e-12)
+
+
+def test_shell_ladder(L, data):
+    _, p, z, fc = data
+    R = np.zeros(64)
+    ze = np.zeros(64)
+    nops = np.zeros(64, np.int32)
+    hom = L.orc_ts_plan_summary(C.byref(p), R.ctypes.data_as(C.c_void_p), ze.ctypes.data_as(C.c_void_p),
+                                nops.ctypes.data_as(C.c_void_p))
+    assert hom == 0
+    R = R[:40]
+    box = p.box_size / p.hubble_h
+    assert R[0] == pytest.approx(0.620350491 * box / 32, rel=1e-6)
+    assert R[-1] * (500 / R[0]) ** (1 / 40) == pytest.approx(500, rel=1e-5)  # R_XLy_MAX after 40 steps
+    assert np.all(np.diff(ze[:40]) > 0)  # light-cone edges recede in redshift
+    assert ze[0] > z[3]
+
+
+def test_no_light_evolution_closed_form():
+    """With no sources the x_e / T_K step is recombination, Compton and adiabatic cooling only."""
+    N, L_ = 16, 25.0
+    z = P.snapshot_redshifts()
+    fc = np.zeros(len(z))
+    d0 = np.zeros((N, N, N), np.float32)
+    zero = synth.to_padded(d0)
+    hist = np.zeros((4, N, N, 2 * (N // 2 + 1)), np.float32)
+    xe0 = np.full((N, N, 2 * (N // 2 + 1)), 3e-4, np.float32)
+    T0 = np.full((N, N, N), 12.0, np.float32)
+    p = P.ts_params(N, L_, 2, z, fc, 4)
+    o = O.compute_ts(p, zero, zero, zero, hist, xe0, T0)
+    assert o["no_light"] == 1 and o["collapse_fraction"] == 0.0
+    lib = O.lib()
+    lib.orc_ts_data_new.restype = C.c_void_p
+    lib.orc_ts_data_new.argtypes = [C.POINTER(P.TsParams)]
+    lib.orc_ts_scalar.restype = C.c_double
+    lib.orc_ts_scalar.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    lib.orc_ts_No.restype = C.c_double
+    lib.orc_ts_No.argtypes = [C.c_void_p, C.c_int]
+    dd = lib.orc_ts_data_new(C.byref(p))
+    zp = float(np.float32(z[2]))
+    dzp = z[2] - z[1]
+    dt_dzp = lib.orc_ts_scalar(dd, 5, z[2])
+    Nb0, fH, fHe = lib.orc_ts_No(dd, 1), lib.orc_ts_No(dd, 2), lib.orc_ts_No(dd, 3)
+    xe, T = 3e-4, 12.0
+    n_b = Nb0 * (1 + zp) ** 3
+    dxe = dt_dzp * (-lib.orc_ts_scalar(dd, 8, T) * 2.0 * xe * xe * fH * n_b)
+    Trad = 2.728 * (1 + zp)
+    Hz = lib.orc_ts_scalar(dd, 6, zp)
+    dcomp = -1.51e-4 * xe / (1 + xe + fHe) / (Hz / (P.HUBBLE * P.HUBBLE_H)) / P.HUBBLE_H * Trad**4 / (1 + zp) * (Trad - T)
+    dT = dcomp + (-dxe * T / (1 + xe)) + 2.0 / (1 + zp) * T
+    assert o["x_e_box_prev"][0, 0, 0] == pytest.approx(xe + dxe * dzp, rel=1e-6)
+    assert o["Tk_box"][0, 0, 0] == pytest.approx(T + dT * dzp, rel=1e-6)
+    # collisions-only spin temperature (get_Ts without Ly-alpha)
+    TK = float(o["Tk_box"][0, 0, 0])
+    xen = float(o["x_e_box_prev"][0, 0, 0])
+    No = lib.orc_ts_No(dd, 0)
+    kHH, keH, kpH = (lib.orc_ts_scalar(dd, k, TK) for k in (2, 3, 4))
+    nz3 = (1 + zp) ** 3
+    xc = 0.0628 / Trad * ((1 - xen) * No * nz3 * kHH + xen * Nb0 * nz3 * keH + xen * No * nz3 * kpH) / 2.85e-15
+    TS = (1 + xc) / (1 / Trad + xc / TK)
+    assert o["TS_box"][0, 0, 0] == pytest.approx(TS, rel=1e-6)
+    lib.orc_ts_data_free(dd)
