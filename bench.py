@@ -1,10 +1,13 @@
 """Benchmark of the excursion-set reionization path on one MI355X.
 
-A step is one snapshot of the hot path on a 512^3, 400 Mpc/h box at z = 8:
-find_HII_bubbles over the full radius ladder (44 radii, top-hat window, UVB on,
-three fields filtered per radius) followed by the brightness-temperature grid,
-all inputs resident in HBM when the timed region starts.  The metric is
-cells*radii per second (N^3 * number of radii / step time).
+A step is one snapshot of the hot path on a 512^3, 400 Mpc/h box, all inputs
+resident in HBM when the timed region starts:
+  --workload snapshot (default): the reference's order per snapshot - the
+      spin-temperature step (40 shells, inhomogeneous branch, SFR history),
+      find_HII_bubbles with x_e as the fourth field (44 radii, top-hat, UVB on),
+      then the brightness-temperature grid with T_S;
+  --workload bubbles: find_HII_bubbles (three fields) + brightness temperature.
+The metric is cells*radii per second (N^3 * bubble radii / step time).
 
 Besides the headline line it reports, from HIP events recorded around every
 kernel launch on the stream the kernels run on, the dominant kernel's average
@@ -48,6 +51,7 @@ def kernel_bytes(name, N, F):
         # strided pencil passes: read + write one spectrum per field
         "colpass_y_inv": 2 * F * spec,
         "colpass_y_fwd": 2 * spec,
+        "ts_cell": 40 * 4 * Nr + 4 * N * N * (N + 2) * 3 + 3 * 4 * Nr,
         "colpass_x_fwd": 2 * spec,
         # window + inverse x pass: read unfiltered (+ Nyquist plane) and write per field
         "ix_filter": F * (2 * spec + 8 * N * N),
@@ -59,25 +63,58 @@ def kernel_bytes(name, N, F):
     return table.get(name)
 
 
-def cpu_baseline(sample_grid=256, box=200.0):
-    """Oracle (ref32) over the full ladder of a 256^3, 200 Mpc/h box (same cell size as 512^3/400)."""
+def cpu_baseline(snapshot):
+    """The oracle (ref32 restatement of the reference's CPU path) on a bounded sample.
+
+    bubbles: find_HII_bubbles over the full ladder of a 256^3, 200 Mpc/h box
+    (the 512^3 / 400 Mpc/h cell size).  snapshot: spin temperature + bubble
+    finder with x_e + brightness temperature on a 160^3, 125 Mpc/h box (same
+    cell size), the same z and history as the GPU step.
+    """
+    import numpy as np
+
     from meraxes_amd import params as P
     from meraxes_amd import synth
     from oracle import oracle as O
 
-    d, s, w = synth.snapshot_inputs(sample_grid, 1)
-    p = P.hii_params(sample_grid, box, 8.0, filter_type=0, efficiency=20.0, uvb_flag=1)
-    t0 = time.perf_counter()
-    r = O.find_hii_bubbles(p, d, s, w)
-    dt = time.perf_counter() - t0
-    S = r["n_radii"]
+    if not snapshot:
+        n, box = 256, 200.0
+        d, s, w = synth.snapshot_inputs(n, 1)
+        p = P.hii_params(n, box, 8.0, filter_type=0, efficiency=20.0, uvb_flag=1)
+        t0 = time.perf_counter()
+        r = O.find_hii_bubbles(p, d, s, w)
+        dt = time.perf_counter() - t0
+        S = r["n_radii"]
+        what = f"find_HII_bubbles, {n}^3 box {box:g} Mpc/h z=8, full ladder S={S}, F=3, UVB on, top-hat"
+    else:
+        n, box = 160, 125.0
+        zz = P.snapshot_redshifts()
+        snap = 24
+        z = float(zz[snap])
+        d, s, w = synth.snapshot_inputs(n, 1)
+        nh = 8
+        hist = np.stack([w * np.float32(((1 + z) / (1 + zz[max(snap - k, 0)])) ** 3) for k in range(nh)])
+        fcoll = np.zeros(len(zz))
+        fcoll[:snap] = np.linspace(1e-4, 5e-3, snap)
+        tsp = P.ts_params(n, box, snap, zz, fcoll, nh, efficiency=20.0)
+        xe0 = np.full_like(d, 2e-3)
+        Tk0 = np.full((n, n, n), 50.0, np.float32)
+        p = P.hii_params(n, box, z, filter_type=0, efficiency=20.0, uvb_flag=1, include_spin_temp=1)
+        tp = P.tb_params(n, z, include_spin_temp=1)
+        t0 = time.perf_counter()
+        ts = O.compute_ts(tsp, d, s, w, hist, xe0, Tk0)
+        r = O.find_hii_bubbles(p, d, s, w, x_e=ts["x_e_box"])
+        O.brightness_temp(tp, r["xH"], d, ts["TS_box"])
+        dt = time.perf_counter() - t0
+        S = r["n_radii"]
+        what = (f"ComputeTs (40 shells) + find_HII_bubbles (F=4, S={S}) + dTb, {n}^3 box {box:g} Mpc/h "
+                f"z={z:.3f}")
     return {
-        "value": sample_grid**3 * S / dt,
+        "value": n**3 * S / dt,
         "unit": "cells*radii/s",
         "cores": O.get_threads(),
         "kind": "port",
-        "sample": f"oracle ref32 find_HII_bubbles, {sample_grid}^3 box {box} Mpc/h z=8, full ladder S={S}, "
-                  f"F=3, UVB on, top-hat; {dt:.1f} s wall",
+        "sample": f"oracle ref32: {what}; {dt:.1f} s wall",
     }
 
 
@@ -88,6 +125,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["snapshot", "bubbles"], default="snapshot")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the reionization path runs the whole box on a single MI355X "
@@ -103,16 +141,19 @@ def main():
 
     N = args.grid
     L = 400.0 * N / 512
-    z = 8.0
-    log(f"[bench] generating {N}^3 inputs")
+    snapshot = args.workload == "snapshot"
+    zz = P.snapshot_redshifts()
+    snap = 24  # z = 8.41 in the 30-snapshot run from z = 20 to 6
+    z = float(zz[snap])
+    log(f"[bench] generating {N}^3 inputs ({args.workload})")
     d, s, w = synth.snapshot_inputs(N, 0)
     dev = torch.device("cuda:0")
     dd, ds, dw = (torch.from_numpy(a).to(dev) for a in (d, s, w))
     del d, s, w
-    p = P.hii_params(N, L, z, filter_type=0, efficiency=20.0, uvb_flag=1)
-    tp = P.tb_params(N, z)
+    F = 4 if snapshot else 3
+    p = P.hii_params(N, L, z, filter_type=0, efficiency=20.0, uvb_flag=1, include_spin_temp=1 if snapshot else 0)
+    tp = P.tb_params(N, z, include_spin_temp=1 if snapshot else 0)
     S = len(P.radius_ladder(p))
-    F = 3
     xH = torch.empty((N, N, N), dtype=torch.float32, device=dev)
     rb = torch.empty_like(xH)
     J21 = torch.empty_like(xH)
@@ -120,12 +161,37 @@ def main():
     zin = torch.full_like(xH, -1.0)
     J21i = torch.zeros_like(xH)
     ctx = Context(0)
+    if snapshot:
+        # carried state and SFR history: earlier snapshots' SFR grids scaled by the
+        # linear growth of the source field, (1+z)^-3 relative to this one
+        nh = 8
+        hist = torch.empty((nh,) + tuple(dw.shape), dtype=torch.float32, device=dev)
+        for k in range(nh):
+            zk = zz[max(snap - k, 0)]
+            hist[k] = dw * float(((1 + z) / (1 + zk)) ** 3)
+        fcoll = np.zeros(len(zz))
+        fcoll[:snap] = np.linspace(1e-4, 5e-3, snap)
+        tsp = P.ts_params(N, L, snap, zz, fcoll, nh, efficiency=20.0)
+        xe0 = torch.full_like(dd, 2e-3)
+        Tk0 = torch.full_like(xH, 50.0)
+        xe_prev = torch.empty_like(dd)
+        Tk = torch.empty_like(xH)
+        xe_box = torch.empty_like(dd)
+        TS = torch.empty_like(xH)
 
     def step():
         zin.fill_(-1.0)
-        r = reion.find_hii_bubbles(ctx, p, dd, ds, dw, z_at_ionization=zin, J_21_at_ionization=J21i,
+        xe_in = None
+        TSb = None
+        if snapshot:
+            xe_prev.copy_(xe0)
+            Tk.copy_(Tk0)
+            reion.compute_ts(ctx, tsp, dd, ds, dw, hist, xe_prev, Tk, x_e_box=xe_box, TS_box=TS)
+            xe_in = xe_box
+            TSb = TS
+        r = reion.find_hii_bubbles(ctx, p, dd, ds, dw, x_e=xe_in, z_at_ionization=zin, J_21_at_ionization=J21i,
                                    out={"xH": xH, "r_bubble": rb, "J_21": J21})
-        reion.brightness_temp(ctx, tp, xH, dd, out=dT)
+        reion.brightness_temp(ctx, tp, xH, dd, TSb, out=dT)
         return r
 
     log(f"[bench] warmup {args.warmup}")
@@ -155,10 +221,14 @@ def main():
     roof = {"kernel": name, "bound": "hbm", "achieved": (b / avg_s / 1e9) if b else None, "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": (b / avg_s / HBM_PEAK) if b else None, "traffic": None,
             "avg_launch_us": avg_s * 1e6, "bytes_per_launch": b}
+    Bc = 8 * N * N * (N // 2 + 1)
     mb = model_bytes_hii(N, F, S, True) + 16 * N**3
+    if snapshot:
+        mb += 40 * (12 * Bc + 8 * N**3) + 184 * N**3  # the 3-pass model of the Ts shells and per-cell pass
     cells_radii = N**3 * S
     out = {
-        "metric": "cells*radii/s of find_HII_bubbles + dTb at 512^3 (plain path)",
+        "metric": ("cells*radii/s of one snapshot (Ts + find_HII_bubbles + dTb) at 512^3, plain path" if snapshot
+                   else "cells*radii/s of find_HII_bubbles + dTb at 512^3, plain path"),
         "value": cells_radii / dt,
         "unit": "cells*radii/s",
         "n_gpus": 1,
@@ -170,9 +240,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"find_HII_bubbles+dTb, {N}^3 grid, {L:g} Mpc/h, z={z}, top-hat, UVB on, "
-                               f"eps=20, {S} radii, {F} fields", "grid": N, "box_mpc_h": L, "radii": S,
-                   "fields": F, "parallelism": "single GPU"},
+        "config": {"workload": (f"{'ComputeTs (40 shells) + ' if snapshot else ''}find_HII_bubbles + dTb, "
+                                f"{N}^3 grid, {L:g} Mpc/h, z={z:.3f}, top-hat, UVB on, eps=20, {S} radii, "
+                                f"{F} fields"), "grid": N, "box_mpc_h": L, "radii": S,
+                   "fields": F, "ts_shells": 40 if snapshot else 0, "parallelism": "single GPU"},
         "model_hbm_frac": mb / dt / HBM_PEAK,
         "model_bytes": mb,
         "roofline": roof,
@@ -180,7 +251,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(snapshot)
         log(f"[bench] CPU baseline {out['cpu_baseline']['value']:.3e} cells*radii/s "
             f"on {out['cpu_baseline']['cores']} threads")
     ctx.close()

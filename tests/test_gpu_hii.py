@@ -45,7 +45,7 @@ def run_pair(ctx, p, d, s, w, xe=None, z_in=None, J21_in=None):
     return g, o
 
 
-def check_parity(g, o, N, label):
+def check_parity(g, o, N, label, eff=20.0):
     dxh, dmass, frac, worst, flips = compare(g, o, N, label)
     assert g["n_radii"] == o["n_radii"]
     assert dxh <= 1e-5
@@ -57,7 +57,9 @@ def check_parity(g, o, N, label):
     ion = (o["xH"] < 1e-5) & keep
     assert np.array_equal(g["xH"][ion], o["xH"][ion])
     neutral = keep & ~ion
-    assert np.allclose(g["xH"][neutral], o["xH"][neutral], rtol=0, atol=5e-5)
+    # partial xH = n_f - eff * f_coll on the unfiltered last step: the float
+    # transforms' ~1e-7 (of the field's rms) error is multiplied by eff
+    assert np.allclose(g["xH"][neutral], o["xH"][neutral], rtol=0, atol=2.5e-6 * eff)
     zi = keep & (o["z_at_ionization"] >= 0)
     assert np.array_equal(g["z_at_ionization"][zi], o["z_at_ionization"][zi])
     return flips
@@ -84,7 +86,7 @@ def test_uvb_and_carried_state(ctx):
     z_in = np.where(rng.random((N, N, N)) < 0.01, np.float32(10.0), np.float32(-1.0)).astype(np.float32)
     J_in = np.where(z_in > 0, np.float32(0.5), np.float32(0.0)).astype(np.float32)
     g, o = run_pair(ctx, p, d, s, w, z_in=z_in, J21_in=J_in)
-    flips = check_parity(g, o, N, "UVB")
+    flips = check_parity(g, o, N, "UVB", eff=60.0)
     keep = ~flips
     both = keep & (o["xH"] < 1e-5)
     rel = np.abs(g["J_21"][both] - o["J_21"][both]) / np.maximum(np.abs(o["J_21"][both]), 1e-30)
@@ -105,7 +107,7 @@ def test_spin_temperature_field(ctx):
     xe = synth.to_padded((0.02 + 0.05 * rng.random((N, N, N))).astype(np.float32))
     p = P.hii_params(N, 100.0, 8.0, efficiency=40.0, include_spin_temp=1)
     g, o = run_pair(ctx, p, d, s, w, xe=xe)
-    check_parity(g, o, N, "x_e")
+    check_parity(g, o, N, "x_e", eff=40.0)
 
 
 def test_config2_like_128_sharpk(ctx):
