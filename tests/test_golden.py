@@ -1,0 +1,59 @@
+"""Regression pins: the oracle reproduces the committed golden fixtures exactly,
+and the HIP path matches them to the parity tolerances (fixtures made by
+tests/golden/make_golden.py)."""
+import os
+
+import numpy as np
+import pytest
+
+from meraxes_amd import params as P
+from meraxes_amd import reion, synth
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(HERE, name)))
+
+
+@pytest.mark.parametrize("ftype", [0, 1, 2])
+def test_oracle_reproduces_hii_fixture(ftype):
+    import sys
+
+    sys.path.insert(0, HERE)
+    from make_golden import hii_case
+
+    arrays, scal = hii_case(ftype=ftype)
+    g = load(f"hii_n32_type{ftype}.npz")
+    for k, v in arrays.items():
+        assert np.array_equal(v, g[k]), k
+    for k, v in scal.items():
+        assert v == g[k], k
+
+
+def test_oracle_reproduces_ts_fixture():
+    import sys
+
+    sys.path.insert(0, HERE)
+    from make_golden import ts_case
+
+    out = ts_case()
+    g = load("ts_n32.npz")
+    for k, v in out.items():
+        assert np.array_equal(v, g[k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ftype", [0, 1, 2])
+def test_gpu_matches_hii_fixture(ctx, ftype):
+    g = load(f"hii_n32_type{ftype}.npz")
+    N = 32
+    d, s, w = synth.snapshot_inputs(N, 0)
+    p = P.hii_params(N, 50.0, 8.0, filter_type=ftype, efficiency=60.0, uvb_flag=1)
+    r = reion.find_hii_bubbles(ctx, p, d, s, w)
+    flips = (r["xH"] < 1e-5) != (g["xH"] < 1e-5)
+    assert flips.mean() <= 1e-4
+    if flips.any():
+        assert np.abs(g["max_ratio"][flips] - 1).max() <= 1e-4
+    assert abs(r["volume_weighted_global_xH"] - float(g["volume_weighted_global_xH"])) <= 1e-5
+    assert r["n_radii"] == int(g["n_radii"])
