@@ -118,6 +118,84 @@ def cpu_baseline(snapshot):
     }
 
 
+BATCH_SIZES = (64, 128, 192)
+BATCH_PER_SIZE = 32
+
+
+def bench_batched(ctx, steps, warmup, per_size=BATCH_PER_SIZE):
+    """96 uneven boxes (32 each of 64^3, 128^3, 192^3, 100 Mpc/h, z=8, top-hat,
+    three fields, efficiency U[10, 40], own seed each): the batch replayed as one
+    graph launch, against the per-box plain loop over the same boxes."""
+    import numpy as np
+    import torch
+
+    from meraxes_amd import params as P
+    from meraxes_amd import reion, synth
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(4)
+    boxes = []
+    cells_radii = 0
+    mbytes = 0
+    log(f"[bench] batched: generating {per_size * len(BATCH_SIZES)} boxes on the GPU")
+    for N in BATCH_SIZES:
+        for i in range(per_size):
+            seed = 1000 * N + i
+            d, s, w = synth.snapshot_inputs_torch(N, seed, dev)
+            p = P.hii_params(N, 100.0, 8.0, filter_type=0, efficiency=float(rng.uniform(10.0, 40.0)))
+            S = len(P.radius_ladder(p))
+            cells_radii += N**3 * S
+            mbytes += model_bytes_hii(N, 3, S, False)
+            boxes.append({"params": p, "deltax": d, "stars": s, "weighted_sfr": w})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch = reion.HiiBatch(ctx, boxes, graph=True, tune=True)
+    t_plan = time.perf_counter() - t0
+    info = batch.describe()
+    log(f"[bench] batched plan {t_plan:.2f} s: {info}")
+    for _ in range(warmup):
+        batch.run(sync=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(sync=False)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    log(f"[bench] batched: {t_batch * 1e3:.2f} ms per graph launch")
+    # the per-box plain loop over the same boxes and outputs
+    for N in BATCH_SIZES:
+        b = next(x for x in batch.boxes if x["params"].grid_dim == N)
+        reion.find_hii_bubbles(ctx, b["params"], b["deltax"], b["stars"], b["weighted_sfr"],
+                               z_at_ionization=b["z_at_ionization"], out=b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in batch.boxes:
+        reion.find_hii_bubbles(ctx, b["params"], b["deltax"], b["stars"], b["weighted_sfr"],
+                               z_at_ionization=b["z_at_ionization"], out=b)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    log(f"[bench] per-box plain loop: {t_plain * 1e3:.2f} ms")
+    batch.close()
+    return {
+        "metric": "cells*radii/s of find_HII_bubbles over 96 uneven boxes, batched (one graph launch)",
+        "value": cells_radii / t_batch,
+        "unit": "cells*radii/s",
+        "ms_per_launch": t_batch * 1e3,
+        "plain_loop_value": cells_radii / t_plain,
+        "plain_loop_ms": t_plain * 1e3,
+        "speedup_vs_plain_loop": t_plain / t_batch,
+        "model_hbm_frac": mbytes / t_batch / HBM_PEAK,
+        "model_bytes": mbytes,
+        "cells_radii": cells_radii,
+        "plan_s": t_plan,
+        "graph_nodes": info["graph_nodes"],
+        "cohorts": [{k: c[k] for k in ("grid_dim", "boxes", "radii", "variant", "tile_width", "radix_schedule")}
+                    for c in info["cohorts"]],
+        "config": {"workload": "32 x 64^3 + 32 x 128^3 + 32 x 192^3 boxes, 100 Mpc/h, z=8, top-hat, 3 fields, "
+                               "eps U[10,40], synthetic (torch generator)"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,7 +203,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["snapshot", "bubbles"], default="snapshot")
+    ap.add_argument("--workload", choices=["snapshot", "bubbles", "batched"], default="snapshot")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched line of the snapshot run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the reionization path runs the whole box on a single MI355X "
@@ -138,6 +217,17 @@ def main():
     from meraxes_amd import params as P
     from meraxes_amd import reion, synth
     from meraxes_amd._lib import Context
+
+    if args.workload == "batched":
+        ctx = Context(0)
+        b = bench_batched(ctx, args.steps, args.warmup)
+        ctx.close()
+        out = {"metric": b["metric"], "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": b["ms_per_launch"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": b["config"],
+               "batched": b}
+        print(json.dumps(out), flush=True)
+        return
 
     N = args.grid
     L = 400.0 * N / 512
@@ -249,6 +339,8 @@ def main():
         "roofline": roof,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
     }
+    if snapshot and not args.no_extras:
+        out["batched"] = bench_batched(ctx, 2, 1)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(snapshot)

@@ -234,8 +234,55 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
 int mrx_profile(mrx_ctx* ctx, int enable);
 int mrx_profile_read(mrx_ctx* ctx, int max_kinds, char* names, int name_len, double* ms_total, long* counts);
 
+/* ---------------------------------------------------------------------------
+ * Batches of independent boxes.  No counterpart in the reference, which runs
+ * one box per call (find_HII_bubbles.c:498-532); each box's results equal
+ * those of mrx_find_hii_bubbles on that box alone.  For callers that push many
+ * small or uneven boxes (parameter sweeps, ensembles) through the bubble
+ * finder: boxes sharing grid_dim, box_size, the ladder (r_bubble_max/min,
+ * delta_r_factor), the filter, the flags and omega_m * rho_crit form a cohort
+ * whose kernels cover all its boxes at once; cohorts run side by side.
+ * efficiency, redshift, gamma_halo_bias, alpha_uv, nion_phot_per_bary and the
+ * units may differ between the boxes of a cohort.
+ *
+ * mrx_hii_batch_create validates the boxes (device pointers only, 16-byte
+ * aligned, the layouts of mrx_find_hii_bubbles), allocates every work buffer,
+ * times the tile-width / radix-schedule variants of each cohort's strided
+ * passes and keeps the fastest (unless MRX_BATCH_NO_TUNE), and captures the
+ * whole computation - every radius of every box - into one HIP graph (unless
+ * MRX_BATCH_NO_GRAPH).  The grids are bound to the plan: mrx_hii_batch_run
+ * reads the current contents of the same device buffers on each call and
+ * replays the graph as one launch on the context stream, then, if `out` is
+ * not NULL, waits and writes one mrx_hii_globals per box (in box order).
+ * mrx_hii_batch_describe writes a JSON summary (cohorts, radii, chosen
+ * variants, measured variant times) and returns its length.
+ */
+typedef struct mrx_hii_batch_box {
+  mrx_hii_params params; /* params.pointers_on_device must be 1 */
+  const float* deltax;
+  const float* stars;
+  const float* weighted_sfr;
+  const float* x_e; /* read when params.include_spin_temp */
+  float* xH;
+  float* r_bubble;
+  float* J_21; /* written when params.uvb_flag */
+  float* z_at_ionization;
+  float* J_21_at_ionization; /* with params.uvb_flag */
+} mrx_hii_batch_box;
+
+typedef struct mrx_hii_batch mrx_hii_batch;
+
+#define MRX_BATCH_NO_GRAPH 1 /* enqueue the kernels on each run instead of replaying a graph */
+#define MRX_BATCH_NO_TUNE 2  /* default tile and radix schedules, nothing measured */
+
+int mrx_hii_batch_create(mrx_ctx* ctx, const mrx_hii_batch_box* boxes, int n_boxes, int flags,
+                         mrx_hii_batch** out);
+int mrx_hii_batch_run(mrx_hii_batch* batch, mrx_hii_globals* out);
+int mrx_hii_batch_describe(const mrx_hii_batch* batch, char* buf, int len);
+int mrx_hii_batch_free(mrx_hii_batch* batch);
+
 /* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params,
- * 3 mrx_ts_params, 4 mrx_ts_globals),
+ * 3 mrx_ts_params, 4 mrx_ts_globals, 5 mrx_hii_batch_box),
  * so a binding can check that its mirror of the layout matches this build. */
 size_t mrx_abi_sizeof(int which);
 

@@ -6,7 +6,7 @@ entry point raises.  Build it with ``python -m meraxes_amd.build``.
 import ctypes as C
 import os
 
-from .abi import ERRORS, HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams
+from .abi import ERRORS, HiiBatchBox, HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmrx_hip.so")
@@ -16,6 +16,7 @@ EXPORTS = (
     "mrx_init", "mrx_free", "mrx_last_error", "mrx_synchronize", "mrx_set_stream", "mrx_radius_ladder",
     "mrx_find_hii_bubbles", "mrx_filter", "mrx_fft_r2c_3d", "mrx_fft_c2r_3d", "mrx_filtered_field",
     "mrx_brightness_temp", "mrx_abi_sizeof", "mrx_profile", "mrx_profile_read", "mrx_compute_ts",
+    "mrx_hii_batch_create", "mrx_hii_batch_run", "mrx_hii_batch_describe", "mrx_hii_batch_free",
 )
 
 _lib = None
@@ -53,6 +54,10 @@ def load():
     L.mrx_compute_ts.argtypes = [ctxp, C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals)]
     L.mrx_profile.argtypes = [ctxp, i]
     L.mrx_profile_read.argtypes = [ctxp, i, C.c_char_p, i, vp, vp]
+    L.mrx_hii_batch_create.argtypes = [ctxp, C.POINTER(HiiBatchBox), i, i, C.POINTER(vp)]
+    L.mrx_hii_batch_run.argtypes = [vp, C.POINTER(HiiGlobals)]
+    L.mrx_hii_batch_describe.argtypes = [vp, C.c_char_p, i]
+    L.mrx_hii_batch_free.argtypes = [vp]
     L.mrx_abi_sizeof.argtypes = [i]
     L.mrx_abi_sizeof.restype = C.c_size_t
     _lib = L

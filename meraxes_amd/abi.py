@@ -142,3 +142,24 @@ def fptr(a):
     if isinstance(a, int):
         return C.c_void_p(a)
     return a.ctypes.data_as(C.c_void_p)
+
+
+MRX_BATCH_NO_GRAPH = 1
+MRX_BATCH_NO_TUNE = 2
+
+
+class HiiBatchBox(C.Structure):
+    """mrx_hii_batch_box: one box of a batch (device grids only)."""
+
+    _fields_ = [
+        ("params", HiiParams),
+        ("deltax", C.c_void_p),
+        ("stars", C.c_void_p),
+        ("weighted_sfr", C.c_void_p),
+        ("x_e", C.c_void_p),
+        ("xH", C.c_void_p),
+        ("r_bubble", C.c_void_p),
+        ("J_21", C.c_void_p),
+        ("z_at_ionization", C.c_void_p),
+        ("J_21_at_ionization", C.c_void_p),
+    ]

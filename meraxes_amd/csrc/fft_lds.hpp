@@ -101,8 +101,17 @@ struct Bfly<8, INV> {
   }
 };
 
+// LDS address of linear element i: with PAD, one float2 of padding every 32
+// elements, so the stride-R writes of a Stockham stage over contiguous rows
+// land in distinct banks.
+template <bool PAD>
+__device__ __forceinline__ int lds_addr(int i)
+{
+  return PAD ? i + (i >> 5) : i;
+}
+
 // One Stockham stage over a tile of NCOL pencils of length L.
-template <int L, int R, int NS, int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS>
+template <int L, int R, int NS, int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS, bool PAD = false>
 __device__ __forceinline__ void stockham_stage(float2* s, const float2* tw, int tid)
 {
   constexpr int NB = NCOL * (L / R);
@@ -115,7 +124,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, const float2* tw, int 
       const int c = CFAST ? (b % NCOL) : (b / (L / R));
       const int j = CFAST ? (b / NCOL) : (b % (L / R));
 #pragma unroll
-      for (int m = 0; m < R; ++m) v[p][m] = s[(j + m * (L / R)) * SN + c * SC];
+      for (int m = 0; m < R; ++m) v[p][m] = s[lds_addr<PAD>((j + m * (L / R)) * SN + c * SC)];
     }
   }
   __syncthreads();
@@ -137,51 +146,69 @@ __device__ __forceinline__ void stockham_stage(float2* s, const float2* tw, int 
       Bfly<R, INV>::run(v[p]);
       const int idx = (NS == 1) ? (j * R) : ((j / NS) * NS * R + k);
 #pragma unroll
-      for (int m = 0; m < R; ++m) s[(idx + m * NS) * SN + c * SC] = v[p][m];
+      for (int m = 0; m < R; ++m) s[lds_addr<PAD>((idx + m * NS) * SN + c * SC)] = v[p][m];
     }
   }
   __syncthreads();
 }
 
-template <int L, int NS, int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS, int R, int... Rest>
+template <int L, int NS, int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS, bool PAD, int R,
+          int... Rest>
 __device__ __forceinline__ void run_stages(float2* s, const float2* tw, int tid)
 {
-  stockham_stage<L, R, NS, NCOL, NT, INV, SN, SC, CFAST, TWS>(s, tw, tid);
-  if constexpr (sizeof...(Rest) > 0) run_stages<L, NS * R, NCOL, NT, INV, SN, SC, CFAST, TWS, Rest...>(s, tw, tid);
+  stockham_stage<L, R, NS, NCOL, NT, INV, SN, SC, CFAST, TWS, PAD>(s, tw, tid);
+  if constexpr (sizeof...(Rest) > 0)
+    run_stages<L, NS * R, NCOL, NT, INV, SN, SC, CFAST, TWS, PAD, Rest...>(s, tw, tid);
 }
 
-// Radix schedules (product = L).  Radix 8 first keeps later strides small.
-template <int L>
-struct Sched;
-#define MRX_SCHED(L_, ...)                                                                                 \
+// Radix schedules (product = L).  Variant 0 puts radix 8 first, which keeps
+// later strides small; variant 1 puts radix 4 first (fewer registers per
+// butterfly, one more stage).  Which is faster depends on the tile shape and
+// is chosen by measurement.
+template <int L, int V>
+struct SchedV;
+#define MRX_SCHED(L_, V_, ...)                                                                             \
   template <>                                                                                             \
-  struct Sched<L_> {                                                                                      \
-    template <int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS>                            \
+  struct SchedV<L_, V_> {                                                                                 \
+    template <int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS, bool PAD>                  \
     __device__ __forceinline__ static void run(float2* s, const float2* tw, int tid)                      \
     {                                                                                                     \
-      run_stages<L_, 1, NCOL, NT, INV, SN, SC, CFAST, TWS, __VA_ARGS__>(s, tw, tid);                      \
+      run_stages<L_, 1, NCOL, NT, INV, SN, SC, CFAST, TWS, PAD, __VA_ARGS__>(s, tw, tid);                 \
     }                                                                                                     \
   };
-MRX_SCHED(16, 4, 4)
-MRX_SCHED(24, 8, 3)
-MRX_SCHED(32, 8, 4)
-MRX_SCHED(48, 8, 2, 3)
-MRX_SCHED(64, 8, 8)
-MRX_SCHED(96, 8, 4, 3)
-MRX_SCHED(128, 8, 4, 4)
-MRX_SCHED(192, 8, 8, 3)
-MRX_SCHED(256, 8, 8, 4)
-MRX_SCHED(384, 8, 4, 4, 3)
-MRX_SCHED(512, 8, 8, 8)
-MRX_SCHED(768, 8, 8, 4, 3)
-MRX_SCHED(1024, 8, 8, 4, 4)
+MRX_SCHED(16, 0, 4, 4)
+MRX_SCHED(24, 0, 8, 3)
+MRX_SCHED(32, 0, 8, 4)
+MRX_SCHED(48, 0, 8, 2, 3)
+MRX_SCHED(64, 0, 8, 8)
+MRX_SCHED(96, 0, 8, 4, 3)
+MRX_SCHED(128, 0, 8, 4, 4)
+MRX_SCHED(192, 0, 8, 8, 3)
+MRX_SCHED(256, 0, 8, 8, 4)
+MRX_SCHED(384, 0, 8, 4, 4, 3)
+MRX_SCHED(512, 0, 8, 8, 8)
+MRX_SCHED(768, 0, 8, 8, 4, 3)
+MRX_SCHED(1024, 0, 8, 8, 4, 4)
+MRX_SCHED(16, 1, 4, 4)
+MRX_SCHED(24, 1, 4, 2, 3)
+MRX_SCHED(32, 1, 4, 8)
+MRX_SCHED(48, 1, 4, 4, 3)
+MRX_SCHED(64, 1, 4, 4, 4)
+MRX_SCHED(96, 1, 4, 8, 3)
+MRX_SCHED(128, 1, 4, 4, 8)
+MRX_SCHED(192, 1, 4, 4, 4, 3)
+MRX_SCHED(256, 1, 4, 4, 4, 4)
+MRX_SCHED(384, 1, 4, 4, 8, 3)
+MRX_SCHED(512, 1, 4, 4, 4, 8)
+MRX_SCHED(768, 1, 4, 4, 4, 4, 3)
+MRX_SCHED(1024, 1, 4, 4, 4, 4, 4)
 #undef MRX_SCHED
 
 // FFT of length L over NCOL pencils in LDS; the table has L*TWS entries.
-template <int L, int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS>
+template <int L, int NCOL, int NT, bool INV, int SN, int SC, bool CFAST, int TWS, bool PAD = false, int V = 0>
 __device__ __forceinline__ void fft_lds(float2* s, const float2* tw, int tid)
 {
-  Sched<L>::template run<NCOL, NT, INV, SN, SC, CFAST, TWS>(s, tw, tid);
+  SchedV<L, V>::template run<NCOL, NT, INV, SN, SC, CFAST, TWS, PAD>(s, tw, tid);
 }
 
 }  // namespace mrx

@@ -71,6 +71,8 @@ struct mrx_ctx {
   size_t ts_work_cap = 0;
   float* ts_shells = nullptr;
   size_t ts_shells_cap = 0;
+  float* wtab = nullptr;  // window table of the current radius, (N/2+1)^3 floats
+  size_t wtab_cap = 0;
 };
 
 namespace {
@@ -284,7 +286,7 @@ struct Pipe {
     // y pass: tile = (x, kz block); rows along y with stride NH
     {
     PROF("colpass_y_fwd");
-    k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
+    k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, (long)N * NH, NH, 0.0);
     }
     MRX_TRY(check_launch(ctx, "k_colpass y"));
     // x pass: tile = (y, kz block); rows along x with stride N*NH
@@ -292,7 +294,7 @@ struct Pipe {
     if (scale && float_scale) div = -div;
     {
     PROF("colpass_x_fwd");
-    k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, NH, (long)N * NH, NH / Cfg<N>::TW, div);
+    k_colpass<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, NH, (long)N * NH, div);
     }
     MRX_TRY(check_launch(ctx, "k_colpass x"));
     const dim3 gu((N * N + 255) / 256, nfield);
@@ -305,15 +307,29 @@ struct Pipe {
   // spectra of the filtered fields with x and y already transformed.
   static int inverse_xy(mrx_ctx* ctx, const float2* U, const float2* V, float2* T, int nfield, const WinArgs& w)
   {
+    const float* Wt = nullptr;
+    if (w.apply) {
+      const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
+      MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt));
+      {
+        PROF("window_table");
+        k_window_table<N><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->wtab, w);
+      }
+      MRX_TRY(check_launch(ctx, "k_window_table"));
+      Wt = ctx->wtab;
+    }
     const dim3 gc(N * (NH / Cfg<N>::TW), 1);
     {
     PROF("ix_filter");
-    k_ix_filter<N><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, nc, (long)N * N, nfield, w);
+    if (Wt)
+      k_ix_filter<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
+    else
+      k_ix_filter<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter"));
     const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
     PROF("colpass_y_inv");
-    k_colpass<N, true><<<gy, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
+    k_colpass<N, true><<<gy, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, 0.0);
     return check_launch(ctx, "k_colpass y inv");
   }
 
@@ -353,9 +369,9 @@ struct Pipe {
   static int inv_xy_plain(mrx_ctx* ctx, float2* T, int nfield)
   {
     const dim3 gc(N * (NH / Cfg<N>::TW), nfield);
-    k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, NH, (long)N * NH, NH / Cfg<N>::TW, 0.0);
+    k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, NH, (long)N * NH, 0.0);
     MRX_TRY(check_launch(ctx, "k_colpass x inv"));
-    k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, NH / Cfg<N>::TW, 0.0);
+    k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, 0.0);
     return check_launch(ctx, "k_colpass y inv");
   }
 };
@@ -762,6 +778,70 @@ int dispatch(int N, A&&... args)
   return dispatch_n<Disp<K>::template F>(N, args...);
 }
 
+#include "batch.hpp"
+
+}  // namespace
+
+struct mrx_hii_batch {
+  mrx_ctx* ctx = nullptr;
+  int nbox = 0;
+  int flags = 0;
+  std::vector<Cohort> cohorts;
+  std::vector<int> box_cohort, box_slot;  // cohort and position of each box
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t fork = nullptr;
+  size_t graph_nodes = 0;
+};
+
+namespace {
+
+void destroy_batch(mrx_hii_batch* b)
+{
+  if (!b) return;
+  if (b->ctx) (void)hipSetDevice(b->ctx->device);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto& c : b->cohorts) {
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    free_cohort(c);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  delete b;
+}
+
+// Enqueues every cohort on its own stream, forked from and joined back into
+// the context stream (captured into the graph when that stream is capturing).
+int enqueue_batch(mrx_ctx* ctx, mrx_hii_batch* b)
+{
+  HIP_TRY(hipEventRecord(b->fork, ctx->stream));
+  for (auto& c : b->cohorts) {
+    HIP_TRY(hipStreamWaitEvent(c.stream, b->fork, 0));
+    CohortRun op{ctx, &c};
+    MRX_TRY(cohort_dispatch(c.N, op));
+    HIP_TRY(hipEventRecord(c.done, c.stream));
+    HIP_TRY(hipStreamWaitEvent(ctx->stream, c.done, 0));
+  }
+  return MRX_OK;
+}
+
+int capture_batch(mrx_ctx* ctx, mrx_hii_batch* b)
+{
+  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_batch(ctx, b);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != MRX_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_err(ctx, MRX_EDEVICE, "batch: graph capture failed: %s", hipGetErrorString(e));
+  b->graph = g;
+  HIP_TRY(hipGraphGetNodes(g, nullptr, &b->graph_nodes));
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return MRX_OK;
+}
+
 }  // namespace
 
 // =============================================================== C ABI
@@ -803,6 +883,7 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->ts_dev);
   dfree(ctx->ts_work);
   dfree(ctx->ts_shells);
+  dfree(ctx->wtab);
   delete ctx->tsd;
   delete ctx->tss;
   delete ctx->ts_host;
@@ -860,6 +941,12 @@ int mrx_find_hii_bubbles(mrx_ctx* ctx, const mrx_hii_params* p, const float* del
   const size_t np = npad(N), nr = nreal(N);
   const float* in[4] = {deltax, stars, weighted_sfr, x_e};
   float* outp[5] = {xH, r_bubble, J_21, z_at_ionization, J_21_at_ionization};
+  if (p->pointers_on_device) {
+    // the kernels move four cells per 16-byte access
+    for (const float* q : {deltax, stars, weighted_sfr, x_e, (const float*)xH, (const float*)r_bubble,
+                           (const float*)J_21, (const float*)z_at_ionization, (const float*)J_21_at_ionization})
+      if (((uintptr_t)q & 15u) != 0) return set_err(ctx, MRX_EINVAL, "device grids must be 16-byte aligned");
+  }
   if (!p->pointers_on_device) {
     // stage: 4 padded inputs + 5 unpadded outputs
     MRX_TRY(ensure_stage(ctx, 4 * np + 5 * nr));
@@ -1145,6 +1232,141 @@ int mrx_profile_read(mrx_ctx* ctx, int max_kinds, char* names, int name_len, dou
   return n;
 }
 
+int mrx_hii_batch_create(mrx_ctx* ctx, const mrx_hii_batch_box* boxes, int n_boxes, int flags,
+                         mrx_hii_batch** out)
+{
+  if (!ctx || !out) return MRX_EINVAL;
+  *out = nullptr;
+  if (!boxes || n_boxes <= 0) return set_err(ctx, MRX_EINVAL, "batch: no boxes");
+  for (int i = 0; i < n_boxes; ++i) {
+    const mrx_hii_batch_box& b = boxes[i];
+    MRX_TRY(validate_hii(ctx, &b.params));
+    if (!b.params.pointers_on_device)
+      return set_err(ctx, MRX_EINVAL, "batch: box %d: grids must be device pointers", i);
+    if (!b.deltax || !b.stars || !b.weighted_sfr || !b.xH || !b.r_bubble || !b.z_at_ionization)
+      return set_err(ctx, MRX_EINVAL, "batch: box %d: a required grid pointer is NULL", i);
+    if (b.params.uvb_flag && (!b.J_21 || !b.J_21_at_ionization))
+      return set_err(ctx, MRX_EINVAL, "batch: box %d: uvb_flag needs J_21 and J_21_at_ionization", i);
+    if (b.params.include_spin_temp && !b.x_e)
+      return set_err(ctx, MRX_EINVAL, "batch: box %d: include_spin_temp needs x_e", i);
+    for (const float* q : {b.deltax, b.stars, b.weighted_sfr, b.x_e, (const float*)b.xH, (const float*)b.r_bubble,
+                           (const float*)b.J_21, (const float*)b.z_at_ionization,
+                           (const float*)b.J_21_at_ionization})
+      if (((uintptr_t)q & 15u) != 0)
+        return set_err(ctx, MRX_EINVAL, "batch: box %d: device grids must be 16-byte aligned", i);
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  mrx_hii_batch* b = new mrx_hii_batch();
+  b->ctx = ctx;
+  b->nbox = n_boxes;
+  b->flags = flags;
+  b->box_cohort.assign(n_boxes, -1);
+  b->box_slot.assign(n_boxes, -1);
+  for (int i = 0; i < n_boxes; ++i) {
+    int k = 0;
+    while (k < (int)b->cohorts.size() && !same_cohort(b->cohorts[k].p0, boxes[i].params)) ++k;
+    if (k == (int)b->cohorts.size()) {
+      b->cohorts.emplace_back();
+      b->cohorts[k].p0 = boxes[i].params;
+    }
+    b->box_cohort[i] = k;
+    b->box_slot[i] = (int)b->cohorts[k].boxes.size();
+    b->cohorts[k].boxes.push_back(i);
+  }
+  int rc = MRX_OK;
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    rc = set_err(ctx, MRX_EDEVICE, "batch: event creation failed");
+  for (auto& c : b->cohorts) {
+    if (rc != MRX_OK) break;
+    c.N = c.p0.grid_dim;
+    c.F = c.p0.include_spin_temp ? 4 : 3;
+    c.nbox = (int)c.boxes.size();
+    c.S = ladder(&c.p0, c.R);
+    if (c.S < 1) rc = set_err(ctx, MRX_EINVAL, "batch: radius ladder is empty");
+    else if (c.S > kMaxSteps)
+      rc = set_err(ctx, MRX_EUNSUPPORTED, "batch: radius ladder has %d steps (max %d)", c.S, kMaxSteps);
+    if (rc != MRX_OK) break;
+    CohortSetup op{ctx, &c, boxes};
+    rc = cohort_dispatch(c.N, op);
+    if (rc == MRX_OK && !(flags & MRX_BATCH_NO_TUNE)) {
+      CohortTune t{ctx, &c};
+      rc = cohort_dispatch(c.N, t);
+    }
+  }
+  if (rc == MRX_OK && !(flags & MRX_BATCH_NO_GRAPH)) rc = capture_batch(ctx, b);
+  if (rc == MRX_OK) {
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = set_err(ctx, MRX_EDEVICE, "batch: %s", hipGetErrorString(e));
+  }
+  if (rc != MRX_OK) {
+    destroy_batch(b);
+    return rc;
+  }
+  *out = b;
+  return MRX_OK;
+}
+
+int mrx_hii_batch_run(mrx_hii_batch* b, mrx_hii_globals* out)
+{
+  if (!b) return MRX_EINVAL;
+  mrx_ctx* ctx = b->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (b->exec)
+    HIP_TRY(hipGraphLaunch(b->exec, ctx->stream));
+  else
+    MRX_TRY(enqueue_batch(ctx, b));
+  if (!out) return finish(ctx);
+  std::vector<std::vector<double>> sums(b->cohorts.size());
+  for (size_t k = 0; k < b->cohorts.size(); ++k) {
+    sums[k].resize(4 * (size_t)b->cohorts[k].nbox);
+    HIP_TRY(hipMemcpyAsync(sums[k].data(), b->cohorts[k].sums, sizeof(double) * sums[k].size(),
+                           hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < b->nbox; ++i) {
+    const Cohort& c = b->cohorts[b->box_cohort[i]];
+    const double* h = sums[b->box_cohort[i]].data() + 4 * (size_t)b->box_slot[i];
+    const double total_n_cells = pow((double)c.N, 3);
+    out[i].volume_weighted_global_xH = h[0] / total_n_cells;
+    out[i].mass_weighted_global_xH = h[1] / h[2];
+    out[i].volume_weighted_global_J_21 = c.p0.uvb_flag ? h[3] / total_n_cells : 0.0;
+    out[i].n_radii = c.S;
+  }
+  return MRX_OK;
+}
+
+int mrx_hii_batch_describe(const mrx_hii_batch* b, char* buf, int len)
+{
+  if (!b || !buf || len <= 0) return MRX_EINVAL;
+  std::string s = "{\"graph\": ";
+  s += b->exec ? "true" : "false";
+  s += ", \"graph_nodes\": " + std::to_string(b->graph_nodes) + ", \"cohorts\": [";
+  for (size_t k = 0; k < b->cohorts.size(); ++k) {
+    const Cohort& c = b->cohorts[k];
+    CohortShape sh{c.variant, 0, 0};
+    (void)cohort_dispatch(c.N, sh);
+    char tmp[256];
+    snprintf(tmp, sizeof tmp, "%s{\"grid_dim\": %d, \"boxes\": %d, \"fields\": %d, \"radii\": %d, \"variant\": %d, "
+             "\"tile_width\": %d, \"radix_schedule\": %d, \"tune_ms\": [", k ? ", " : "", c.N, c.nbox, c.F, c.S,
+             c.variant, sh.tw, sh.sv);
+    s += tmp;
+    for (size_t v = 0; v < c.tune_ms.size(); ++v) {
+      snprintf(tmp, sizeof tmp, "%s%.4f", v ? ", " : "", c.tune_ms[v]);
+      s += tmp;
+    }
+    s += "]}";
+  }
+  s += "]}";
+  snprintf(buf, (size_t)len, "%s", s.c_str());
+  return (int)s.size();
+}
+
+int mrx_hii_batch_free(mrx_hii_batch* b)
+{
+  destroy_batch(b);
+  return MRX_OK;
+}
+
 // Size of each ABI struct, so bindings can check their mirrors.
 size_t mrx_abi_sizeof(int which)
 {
@@ -1154,6 +1376,7 @@ size_t mrx_abi_sizeof(int which)
     case 2: return sizeof(mrx_tb_params);
     case 3: return sizeof(mrx_ts_params);
     case 4: return sizeof(mrx_ts_globals);
+    case 5: return sizeof(mrx_hii_batch_box);
     default: return 0;
   }
 }

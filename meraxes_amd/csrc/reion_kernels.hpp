@@ -45,7 +45,8 @@ struct Cfg {
   static constexpr int ZE = RZ * NH;                               // complex elements per z workgroup
   static constexpr int CPT = 2 * ZE / kNT;                          // real cells per thread in the z pass
   static_assert((N * N) % RZ == 0, "z workgroups must tile the rows");
-  static_assert((2 * ZE) % kNT == 0, "z cells must split evenly over the threads");
+  static_assert(ZE % kNT == 0, "z elements must split evenly over the threads");
+  static_assert(CPT % 4 == 0, "z-pass threads own whole groups of 4 cells");
   static_assert((N * TW) % COL_NT == 0, "strided tiles must split evenly over the threads");
 };
 
@@ -118,36 +119,41 @@ __device__ __forceinline__ float window(int nx, int ny, int nz, int N, const Win
 
 // ------------------------------------------------------------- z: R2C pass
 // in: padded real rows (N+2 floats); out: packed half spectrum rows (N/2).
+// Workgroup blockIdx.x transforms rows [blockIdx.x * RZ, +RZ) of one field.
 template <int N>
-__global__ __launch_bounds__(kNT) void k_r2c_z(const float* __restrict__ in, float2* __restrict__ out,
-                                              const float2* __restrict__ twN, long in_fstride, long out_fstride)
+__device__ __forceinline__ void r2c_z_tile(const float* __restrict__ in, float2* __restrict__ dst,
+                                           const float2* __restrict__ twN)
 {
   using C = Cfg<N>;
   constexpr int NH = C::NH, RZ = C::RZ, ZE = C::ZE;
-  __shared__ float2 s[ZE];
+  __shared__ float2 s[ZE + ZE / 32];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += kNT) tw[i] = twN[i];
   const long row0 = (long)blockIdx.x * RZ;
-  const float2* src = reinterpret_cast<const float2*>(in + blockIdx.y * in_fstride);
-  float2* dst = out + blockIdx.y * out_fstride;
-  for (int e = tid; e < ZE; e += kNT) {
-    const int r = e / NH, n = e % NH;
-    s[e] = src[(row0 + r) * (NH + 1) + n];
+  const float2* src = reinterpret_cast<const float2*>(in);
+  constexpr int EPZ = ZE / kNT;
+  float2 v[EPZ];
+#pragma unroll
+  for (int i = 0; i < EPZ; ++i) {
+    const int e = tid + i * kNT, r = e / NH, n = e % NH;
+    v[i] = src[(row0 + r) * (NH + 1) + n];
   }
+#pragma unroll
+  for (int i = 0; i < EPZ; ++i) s[lds_addr<true>(tid + i * kNT)] = v[i];
   __syncthreads();
-  fft_lds<NH, RZ, kNT, false, 1, NH, false, 2>(s, tw, tid);
+  fft_lds<NH, RZ, kNT, false, 1, NH, false, 2, true>(s, tw, tid);
   // twist: X[k] = E + w^k O, X[NH-k] = conj(E - w^k O), E=(Z_k+conj Z_{NH-k})/2, O=(Z_k-conj Z_{NH-k})/(2i)
   constexpr int NP = NH / 2 + 1;
   for (int q = tid; q < RZ * NP; q += kNT) {
     const int r = q / NP, k = q % NP;
-    const float2* row = s + r * NH;
+    const int rb = r * NH;
     float2* orow = dst + (row0 + r) * NH;
     if (k == 0) {
-      const float2 z = row[0];
+      const float2 z = s[lds_addr<true>(rb)];
       orow[0] = make_float2(z.x + z.y, z.x - z.y);
     } else {
-      const float2 zk = row[k], zm = cconj(row[NH - k]);
+      const float2 zk = s[lds_addr<true>(rb + k)], zm = cconj(s[lds_addr<true>(rb + NH - k)]);
       const float2 E = cscale(cadd(zk, zm), 0.5f);
       const float2 d = csub(zk, zm);
       const float2 O = make_float2(0.5f * d.y, -0.5f * d.x);
@@ -158,34 +164,72 @@ __global__ __launch_bounds__(kNT) void k_r2c_z(const float* __restrict__ in, flo
   }
 }
 
+// blockIdx.y selects the field (strides in_fstride, out_fstride)
+template <int N>
+__global__ __launch_bounds__(kNT) void k_r2c_z(const float* __restrict__ in, float2* __restrict__ out,
+                                              const float2* __restrict__ twN, long in_fstride, long out_fstride)
+{
+  r2c_z_tile<N>(in + blockIdx.y * in_fstride, out + blockIdx.y * out_fstride, twN);
+}
+
+// blockIdx.y selects the field through a table of input pointers (batches of
+// separately allocated boxes); outputs are contiguous
+template <int N>
+__global__ __launch_bounds__(kNT) void k_r2c_z_tab(const float* const* __restrict__ in_tab, float2* __restrict__ out,
+                                                  const float2* __restrict__ twN, long out_fstride)
+{
+  r2c_z_tile<N>(in_tab[blockIdx.y], out + blockIdx.y * out_fstride, twN);
+}
+
 // ---------------------------------------------------- z: C2R load + FFT
 // Loads RZ packed rows, untwists them and runs the inverse length-N/2 FFT in
 // LDS; afterwards the LDS tile, read as floats, holds RZ real rows of N.
+template <int N>
+__device__ __forceinline__ void c2r_untwist_fft(float2* s, const float2* tw, int tid)
+{
+  using C = Cfg<N>;
+  constexpr int NH = C::NH, RZ = C::RZ;
+  constexpr int NP = NH / 2 + 1;
+  for (int q = tid; q < RZ * NP; q += kNT) {
+    const int r = q / NP, k = q % NP;
+    const int rb = r * NH;
+    if (k == 0) {
+      const float2 p = s[lds_addr<true>(rb)];  // (X_0, X_{N/2}), both real
+      s[lds_addr<true>(rb)] = make_float2(p.x + p.y, p.x - p.y);
+    } else {
+      const float2 xk = s[lds_addr<true>(rb + k)], xm = cconj(s[lds_addr<true>(rb + NH - k)]);
+      const float2 E = cadd(xk, xm);
+      const float2 O = cmul(csub(xk, xm), cconj(tw[k]));  // * exp(+2 pi i k / N)
+      s[lds_addr<true>(rb + k)] = make_float2(E.x - O.y, E.y + O.x);  // E + i O
+      if (k != NH - k) s[lds_addr<true>(rb + NH - k)] = make_float2(E.x + O.y, -E.y + O.x);  // conj(E) + i conj(O)
+    }
+  }
+  __syncthreads();
+  fft_lds<NH, RZ, kNT, true, 1, NH, false, 2, true>(s, tw, tid);
+}
+
+// Loads RZ packed rows, untwists them and runs the inverse length-N/2 FFT in
+// LDS; afterwards real cell e of the tile is lds_real<N>(s, e).
 template <int N>
 __device__ __forceinline__ void c2r_rows_lds(float2* s, const float2* tw, const float2* __restrict__ src, long row0,
                                              int tid)
 {
   using C = Cfg<N>;
-  constexpr int NH = C::NH, RZ = C::RZ, ZE = C::ZE;
-  for (int e = tid; e < ZE; e += kNT) s[e] = src[row0 * NH + e];
+  constexpr int NH = C::NH, ZE = C::ZE;
+  constexpr int EPZ = ZE / kNT;
+  float2 v[EPZ];
+#pragma unroll
+  for (int i = 0; i < EPZ; ++i) v[i] = src[row0 * NH + tid + i * kNT];
+#pragma unroll
+  for (int i = 0; i < EPZ; ++i) s[lds_addr<true>(tid + i * kNT)] = v[i];
   __syncthreads();
-  constexpr int NP = NH / 2 + 1;
-  for (int q = tid; q < RZ * NP; q += kNT) {
-    const int r = q / NP, k = q % NP;
-    float2* row = s + r * NH;
-    if (k == 0) {
-      const float2 p = row[0];  // (X_0, X_{N/2}), both real
-      row[0] = make_float2(p.x + p.y, p.x - p.y);
-    } else {
-      const float2 xk = row[k], xm = cconj(row[NH - k]);
-      const float2 E = cadd(xk, xm);
-      const float2 O = cmul(csub(xk, xm), cconj(tw[k]));  // * exp(+2 pi i k / N)
-      row[k] = make_float2(E.x - O.y, E.y + O.x);       // E + i O
-      if (k != NH - k) row[NH - k] = make_float2(E.x + O.y, -E.y + O.x);  // conj(E) + i conj(O)
-    }
-  }
-  __syncthreads();
-  fft_lds<NH, RZ, kNT, true, 1, NH, false, 2>(s, tw, tid);
+  c2r_untwist_fft<N>(s, tw, tid);
+}
+
+__device__ __forceinline__ float lds_real(const float2* s, int e)
+{
+  const float2 v = s[lds_addr<true>(e >> 1)];
+  return (e & 1) ? v.y : v.x;
 }
 
 // aliasing clamps of find_HII_bubbles.c:272-311
@@ -216,101 +260,164 @@ __global__ __launch_bounds__(kNT) void k_c2r_z(const float2* __restrict__ in, fl
 {
   using C = Cfg<N>;
   constexpr int RZ = C::RZ, ZE = C::ZE;
-  __shared__ float2 s[ZE];
+  __shared__ float2 s[ZE + ZE / 32];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += kNT) tw[i] = twN[i];
   const long row0 = (long)blockIdx.x * RZ;
   c2r_rows_lds<N>(s, tw, in + blockIdx.y * in_fstride, row0, tid);
-  const float* sf = reinterpret_cast<const float*>(s);
   float* o = out + blockIdx.y * out_fstride;
   for (int e = tid; e < RZ * pitch; e += kNT) {
     const int r = e / pitch, z = e % pitch;
-    o[(row0 + r) * pitch + z] = (z < N) ? clamp_kind(sf[r * N + z], clamp) : 0.0f;
+    o[(row0 + r) * pitch + z] = (z < N) ? clamp_kind(lds_real(s, r * N + z), clamp) : 0.0f;
   }
 }
 
 // --------------------------------------------------- strided pencil passes
-// One N x kTW tile: pencils along an axis with element stride `rstride`;
-// tile t covers kz block (t mod nkzb) of line (t / nkzb), line stride astride.
-template <int N, bool INV>
-__global__ __launch_bounds__(Cfg<N>::COL_NT) void k_colpass(float2* __restrict__ data, const float2* __restrict__ twN,
-                                                           long fstride, long astride, long rstride, int nkzb,
-                                                           double div)
+// Thread map of an N x TW tile: thread t owns column t % TW and rows
+// t / TW + i * RPI, i < EPT.  Every load of a tile is issued before the first
+// is consumed, so a wave keeps EPT row segments in flight.
+template <int N, int TW_>
+struct TileMap {
+  static constexpr int TW = TW_;
+  static constexpr int NT_RAW = TW * N / 8;
+  static constexpr int NT = NT_RAW < 64 ? 64 : (NT_RAW > kNT ? kNT : NT_RAW);
+  static constexpr int SP = TW + 2;  // LDS row pitch, breaks the bank pattern
+  static constexpr int RPI = NT / TW;
+  static constexpr int EPT = N / RPI;
+  static constexpr int NKZB = (N / 2) / TW;  // kz blocks per line
+  static_assert((N / 2) % TW == 0, "tile width must divide N/2");
+  static_assert(NT % TW == 0 && N % RPI == 0, "tile must split evenly over the threads");
+};
+
+// One N x TW tile per workgroup: pencils along an axis with element stride
+// `rstride`; tile t covers kz block (t mod NKZB) of line (t / NKZB), line
+// stride astride; blockIdx.y selects the field (stride fstride).
+template <int N, bool INV, int TW = Cfg<N>::TW, int SV = 0>
+__global__ __launch_bounds__((TileMap<N, TW>::NT)) void k_colpass(float2* __restrict__ data,
+                                                               const float2* __restrict__ twN, long fstride,
+                                                               long astride, long rstride, double div)
 {
-  constexpr int NT = Cfg<N>::COL_NT, kTW = Cfg<N>::TW;
-  __shared__ float2 s[N * kTW];
+  using M = TileMap<N, TW>;
+  constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
+  __shared__ float2 s[N * SP];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
+  const int c = tid % TW, n0 = tid / TW;
+  const int a = blockIdx.x / NKZB, b = blockIdx.x % NKZB;
+  const unsigned rs = (unsigned)rstride;
+  float2* g = data + blockIdx.y * fstride + (long)a * astride + (long)b * TW + (unsigned)n0 * rs + c;
+  float2 r[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) r[i] = g[(unsigned)(i * RPI) * rs];
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
-  float2* g = data + blockIdx.y * fstride;
-  const int a = blockIdx.x / nkzb, b = blockIdx.x % nkzb;
-  const long base = (long)a * astride + (long)b * kTW;
-  for (int e = tid; e < N * kTW; e += NT) {
-    const int n = e / kTW, c = e % kTW;
-    s[e] = g[base + n * rstride + c];
-  }
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = r[i];
   __syncthreads();
-  fft_lds<N, kTW, NT, INV, kTW, 1, true, 1>(s, tw, tid);
-  for (int e = tid; e < N * kTW; e += NT) {
-    const int n = e / kTW, c = e % kTW;
-    float2 v = s[e];
-    if (div > 0.0) {  // complex float / double, in double (find_HII_bubbles.c:168-182)
-      v = make_float2((float)((double)v.x / div), (float)((double)v.y / div));
-    } else if (div < 0.0) {  // complex float / float (ComputeTs.c:383-384)
-      const float fd = (float)(-div);
-      v = make_float2(v.x / fd, v.y / fd);
-    }
-    g[base + n * rstride + c] = v;
+  fft_lds<N, TW, NT, INV, SP, 1, true, 1, false, SV>(s, tw, tid);
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) r[i] = s[(n0 + i * RPI) * SP + c];
+  if (div > 0.0) {  // complex float / double, in double (find_HII_bubbles.c:168-182)
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) r[i] = make_float2((float)((double)r[i].x / div), (float)((double)r[i].y / div));
+  } else if (div < 0.0) {  // complex float / float (ComputeTs.c:383-384)
+    const float fd = (float)(-div);
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) r[i] = make_float2(r[i].x / fd, r[i].y / fd);
   }
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) g[(unsigned)(i * RPI) * rs] = r[i];
+}
+
+// Window table of one radius: W depends on (|nx|, |ny|, nz) only, because
+// filter() builds each k component as n*dk or (n-N)*dk and squares it, so
+// modes n and N-n give the same float k^2.  Entry (a, b, c) holds
+// W(a, b, c) for a, b, c in [0, N/2].
+template <int N>
+__global__ __launch_bounds__(kNT) void k_window_table(float* __restrict__ Wt, WinArgs w)
+{
+  constexpr int NH1 = N / 2 + 1;
+  const long idx = (long)blockIdx.x * kNT + threadIdx.x;
+  if (idx >= (long)NH1 * NH1 * NH1) return;
+  const int c = (int)(idx % NH1);
+  const long ab = idx / NH1;
+  const int b = (int)(ab % NH1), a = (int)(ab / NH1);
+  Wt[idx] = window(a, b, c, N, w);
 }
 
 // Inverse x pass with the window applied on load, for every field of the
 // ladder step.  U: unfiltered packed spectra (kz=0 slot = true kz=0 plane);
-// V: their Nyquist planes [ky][kx]; T: output packed spectra.
-template <int N>
-__global__ __launch_bounds__(Cfg<N>::COL_NT) void k_ix_filter(const float2* __restrict__ U,
-                                                             const float2* __restrict__ V, float2* __restrict__ T,
-                                                             const float2* __restrict__ twN, long fstride,
-                                                             long vstride, int nfield, WinArgs w)
+// V: their Nyquist planes [ky][kx]; T: output packed spectra; Wt: the
+// radius's window table (WIN), unused for the unfiltered last step.  The
+// tile's window values are loaded once for all fields, and the next field's
+// tile is loaded while the current one is transformed.  blockIdx.y selects a
+// box of a batch (strides bstride for U/T, bvstride for V).
+template <int N, bool WIN, int TW = Cfg<N>::TW, int SV = 0>
+__global__ __launch_bounds__((TileMap<N, TW>::NT)) void k_ix_filter(const float2* __restrict__ U,
+                                                                 const float2* __restrict__ V,
+                                                                 float2* __restrict__ T,
+                                                                 const float2* __restrict__ twN,
+                                                                 const float* __restrict__ Wt, long fstride,
+                                                                 long vstride, int nfield, long bstride,
+                                                                 long bvstride)
 {
-  constexpr int NT = Cfg<N>::COL_NT, NH = N / 2, kTW = Cfg<N>::TW, nkzb = NH / kTW;
-  constexpr int EPT = N * kTW / NT;
-  __shared__ float2 s[N * kTW];
+  using M = TileMap<N, TW>;
+  constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
+  constexpr int NH = N / 2, NH1 = NH + 1;
+  constexpr unsigned rs = (unsigned)N * NH;
+  __shared__ float2 s[N * SP];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += NT) tw[i] = twN[i];
-  const int ky = blockIdx.x / nkzb, b = blockIdx.x % nkzb;
-  const long base = (long)ky * NH + (long)b * kTW;
-  constexpr long rstride = (long)N * NH;
-  float wv[EPT], wn[EPT];
+  const int c = tid % TW, n0 = tid / TW;
+  const int ky = blockIdx.x / NKZB, b = blockIdx.x % NKZB;
+  const int aky = ky <= NH ? ky : N - ky;
+  U += blockIdx.y * bstride;
+  T += blockIdx.y * bstride;
+  V += blockIdx.y * bvstride;
+  const long base = (long)ky * NH + (long)b * TW + (unsigned)n0 * rs + c;
+  float2 r[EPT];
+  {
+    const float2* u = U + base;
 #pragma unroll
-  for (int i = 0; i < EPT; ++i) {
-    const int e = tid + i * NT, n = e / kTW, c = e % kTW;
-    wv[i] = window(n, ky, b * kTW + c, N, w);
-    wn[i] = (b == 0 && c == 0) ? window(n, ky, NH, N, w) : 0.0f;
+    for (int i = 0; i < EPT; ++i) r[i] = u[(unsigned)(i * RPI) * rs];
   }
-  for (int f = 0; f < nfield; ++f) {
-    const float2* u = U + f * fstride;
-    const float2* vp = V + f * vstride + (long)ky * N;
+  // W(|kx|, |ky|, kz) = Wt[(|kx| * NH1 + |ky|) * NH1 + kz]
+  float w[WIN ? EPT : 1];
+  if constexpr (WIN) {
+    const float* wr = Wt + (long)aky * NH1 + b * TW + c;
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
-      const int e = tid + i * NT, n = e / kTW, c = e % kTW;
-      float2 v = cscale(u[base + n * rstride + c], wv[i]);
-      if (b == 0 && c == 0) {
-        const float2 nb = cscale(vp[n], wn[i]);
-        v = make_float2(v.x - nb.y, v.y + nb.x);  // A*W0 + i*(B*WN)
+      const int n = n0 + i * RPI;
+      const int an = n <= NH ? n : N - n;
+      w[i] = wr[(unsigned)an * (NH1 * NH1)];
+    }
+  }
+  for (int i = tid; i < N; i += NT) tw[i] = twN[i];
+  for (int f = 0; f < nfield; ++f) {
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = WIN ? cscale(r[i], w[i]) : r[i];
+    if (b == 0) {
+      // the kz=0 slot carries A*W0 + i*(B*WN), B the Nyquist plane
+      __syncthreads();
+      const float2* vp = V + f * vstride + (long)ky * N;
+      for (int n = tid; n < N; n += NT) {
+        const int an = n <= NH ? n : N - n;
+        const float wn = WIN ? Wt[((unsigned)an * NH1 + aky) * NH1 + NH] : 1.0f;
+        const float2 nb = cscale(vp[n], wn);
+        const float2 v = s[n * SP];
+        s[n * SP] = make_float2(v.x - nb.y, v.y + nb.x);
       }
-      s[e] = v;
     }
     __syncthreads();
-    fft_lds<N, kTW, NT, true, kTW, 1, true, 1>(s, tw, tid);
-    float2* t = T + f * fstride;
+    if (f + 1 < nfield) {
+      const float2* u = U + (f + 1) * fstride + base;
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int e = tid + i * NT, n = e / kTW, c = e % kTW;
-      t[base + n * rstride + c] = s[e];
+      for (int i = 0; i < EPT; ++i) r[i] = u[(unsigned)(i * RPI) * rs];
     }
+    fft_lds<N, TW, NT, true, SP, 1, true, 1, false, SV>(s, tw, tid);
+    float2* t = T + f * fstride + base;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) t[(unsigned)(i * RPI) * rs] = s[(n0 + i * RPI) * SP + c];
     __syncthreads();
   }
 }
@@ -403,11 +510,14 @@ __device__ __forceinline__ double block_sum(double v, double* red)
   return r;
 }
 
-// partials[nb][nv] -> out[nv], fixed order (deterministic)
+// partials[nb][nv] -> out[nv], fixed order (deterministic); block b reduces
+// the b-th such set (one box of a batch)
 __global__ __launch_bounds__(kNT) void k_reduce_partials(const double* __restrict__ partials, int nb, int nv,
                                                        double* __restrict__ out)
 {
   __shared__ double red[kNT];
+  partials += (long)blockIdx.x * nb * nv;
+  out += (long)blockIdx.x * nv;
   for (int v = 0; v < nv; ++v) {
     double acc = 0.0;
     for (int i = threadIdx.x; i < nb; i += kNT) acc += partials[(long)i * nv + v];
@@ -439,33 +549,55 @@ struct StepArgs {
 // r_bubble and the J_21 zeros are written once, on the last step, and the
 // first crossing writes J_21 / z_at_ionization / J_21_at_ionization directly.
 template <int N>
-__global__ __launch_bounds__(kNT) void k_c2r_threshold(const float2* __restrict__ T, long fstride,
-                                                      const float2* __restrict__ twN,
-                                                      const float* __restrict__ deltax_in,
-                                                      uint8_t* __restrict__ state, float* __restrict__ xH,
-                                                      float* __restrict__ r_bubble, float* __restrict__ J21,
-                                                      float* __restrict__ z_in, float* __restrict__ J21_ion,
-                                                      const float* __restrict__ Rtab, double* __restrict__ partials,
-                                                      StepArgs a)
+__device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T, long fstride,
+                                                   const float2* __restrict__ twN,
+                                                   const float* __restrict__ deltax_in,
+                                                   uint8_t* __restrict__ state, float* __restrict__ xH,
+                                                   float* __restrict__ r_bubble, float* __restrict__ J21,
+                                                   float* __restrict__ z_in, float* __restrict__ J21_ion,
+                                                   const float* __restrict__ Rtab, double* __restrict__ partials,
+                                                   const StepArgs& a)
 {
 #pragma clang fp contract(off)
   using C = Cfg<N>;
-  constexpr int RZ = C::RZ, ZE = C::ZE, CPT = C::CPT;
-  __shared__ float2 s[ZE];
+  constexpr int NH = C::NH, RZ = C::RZ, ZE = C::ZE, CPT = C::CPT, EPZ = ZE / kNT;
+  __shared__ float2 s[ZE + ZE / 32];
   __shared__ float2 tw[N];
   __shared__ double red[kNT];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += kNT) tw[i] = twN[i];
   const long row0 = (long)blockIdx.x * RZ;
-  const float* sf = reinterpret_cast<const float*>(s);
+  // issue every field's loads up front so they overlap the transforms
+  float2 pre[4][EPZ];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+    if (f < a.nfield) {
+      const float2* src = T + f * fstride + row0 * NH;
+#pragma unroll
+      for (int i = 0; i < EPZ; ++i) pre[f][i] = src[tid + i * kNT];
+    }
+  // thread t owns the cell groups 4 * (t + j * kNT) + q, q < 4, j < CPT / 4:
+  // four consecutive cells of one row, so state is one word and the outputs
+  // are 16-byte stores
+  constexpr int G = CPT / 4;
   float val[4][CPT];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     if (f < a.nfield) {
-      c2r_rows_lds<N>(s, tw, T + f * fstride, row0, tid);
+#pragma unroll
+      for (int i = 0; i < EPZ; ++i) s[lds_addr<true>(tid + i * kNT)] = pre[f][i];
+      __syncthreads();
+      c2r_untwist_fft<N>(s, tw, tid);
       const int kind = (f == 0) ? 0 : (f == 3 ? 2 : 1);
 #pragma unroll
-      for (int i = 0; i < CPT; ++i) val[f][i] = clamp_kind(sf[tid + i * kNT], kind);
+      for (int j = 0; j < G; ++j) {
+        const int h = 2 * (tid + j * kNT);  // float2 index of the group's first pair
+        const float2 p0 = s[lds_addr<true>(h)], p1 = s[lds_addr<true>(h + 1)];
+        val[f][4 * j + 0] = clamp_kind(p0.x, kind);
+        val[f][4 * j + 1] = clamp_kind(p0.y, kind);
+        val[f][4 * j + 2] = clamp_kind(p1.x, kind);
+        val[f][4 * j + 3] = clamp_kind(p1.y, kind);
+      }
       __syncthreads();
     } else {
 #pragma unroll
@@ -475,54 +607,88 @@ __global__ __launch_bounds__(kNT) void k_c2r_threshold(const float2* __restrict_
   double s_xH = 0.0, s_xHm = 0.0, s_m = 0.0, s_J = 0.0;
   const long cell0 = row0 * N;
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) {
-    const int e = tid + i * kNT;
-    const long ir = cell0 + e;
-    const double density_over_mean = 1.0 + (double)val[0][i];
-    const double f_coll_stars =
-      (double)val[1][i] / (a.M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume;
-    const double weighted_sfr_density = (double)val[2][i] / a.pixel_volume;
-    const double neutral_fraction = a.use_xe ? 1.0 - (double)val[3][i] : 1.0;
-    const float J_21_aux = a.uvb ? (float)(weighted_sfr_density * a.J21c) : 0.0f;
-    const bool cross = f_coll_stars * a.eff > neutral_fraction * (1. + 0.0);
-    const uint8_t st = (a.step == 0) ? (uint8_t)0xFF : state[ir];
-    const bool never = (st == 0xFF);
-    bool zcheck = false;
-    if (cross) {
-      if (never) {
-        if (a.uvb) J21[ir] = J_21_aux;
-        zcheck = true;
+  for (int j = 0; j < G; ++j) {
+    const int e0 = 4 * (tid + j * kNT);
+    const long ir0 = cell0 + e0;
+    const uint32_t st_old = (a.step == 0) ? 0xFFFFFFFFu : *reinterpret_cast<const uint32_t*>(state + ir0);
+    uint32_t st_new = st_old;
+    float xh4[4], rb4[4], j4[4];
+    int zcheck4 = 0;
+    float jaux4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = 4 * j + q;
+      const long ir = ir0 + q;
+      const double density_over_mean = 1.0 + (double)val[0][i];
+      const double f_coll_stars =
+        (double)val[1][i] / (a.M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume;
+      const double weighted_sfr_density = (double)val[2][i] / a.pixel_volume;
+      const double neutral_fraction = a.use_xe ? 1.0 - (double)val[3][i] : 1.0;
+      const float J_21_aux = a.uvb ? (float)(weighted_sfr_density * a.J21c) : 0.0f;
+      jaux4[q] = J_21_aux;
+      const bool cross = f_coll_stars * a.eff > neutral_fraction * (1. + 0.0);
+      const uint32_t st = (st_old >> (8 * q)) & 0xFFu;
+      const bool never = (st == 0xFFu);
+      bool zcheck = false;
+      if (cross) {
+        if (never) {
+          if (a.uvb) J21[ir] = J_21_aux;
+          zcheck = true;
+        }
+        st_new = (st_new & ~(0xFFu << (8 * q))) | ((uint32_t)a.step << (8 * q));
       }
-      state[ir] = (uint8_t)a.step;
-    } else if (a.step == 0) {
-      state[ir] = (uint8_t)0xFF;
+      if (a.last) {
+        const bool ion = cross || !never;
+        float xh;
+        if (ion) {
+          xh = 0.0f;
+        } else {
+          xh = (float)(neutral_fraction - f_coll_stars * a.eff);
+          if (xh < 0.f)
+            xh = 0.f;
+          else if (xh > 1.0f)
+            xh = 1.f;
+          if (xh < kRelTol) zcheck = true;
+        }
+        xh4[q] = xh;
+        rb4[q] = ion ? Rtab[cross ? a.step : (int)st] : 0.0f;
+        j4[q] = ion ? 1.0f : 0.0f;  // marks the cells whose J_21 is kept
+      }
+      zcheck4 |= (zcheck ? 1 : 0) << q;
     }
+    if (st_new != st_old || a.step == 0) *reinterpret_cast<uint32_t*>(state + ir0) = st_new;
     if (a.last) {
-      const bool ion = cross || !never;
-      float xh;
-      if (ion) {
-        xh = 0.0f;
-      } else {
-        xh = (float)(neutral_fraction - f_coll_stars * a.eff);
-        if (xh < 0.f)
-          xh = 0.f;
-        else if (xh > 1.0f)
-          xh = 1.f;
-        if (xh < kRelTol) zcheck = true;
+      *reinterpret_cast<float4*>(xH + ir0) = make_float4(xh4[0], xh4[1], xh4[2], xh4[3]);
+      *reinterpret_cast<float4*>(r_bubble + ir0) = make_float4(rb4[0], rb4[1], rb4[2], rb4[3]);
+      const int r = e0 / N, z = e0 % N;
+      const float* dxp = deltax_in + (row0 + r) * (N + 2) + z;
+      const float2 d01 = *reinterpret_cast<const float2*>(dxp);
+      const float2 d23 = *reinterpret_cast<const float2*>(dxp + 2);
+      const float dx4[4] = {d01.x, d01.y, d23.x, d23.y};
+      float4 jv = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (a.uvb) jv = *reinterpret_cast<const float4*>(J21 + ir0);
+      const float jj[4] = {jv.x, jv.y, jv.z, jv.w};
+      float jo[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double dm = 1.0 + (double)dx4[q];
+        s_xH += (double)xh4[q];
+        s_xHm += (double)xh4[q] * dm;
+        s_m += dm;
+        jo[q] = (j4[q] != 0.0f) ? jj[q] : 0.0f;
+        if (a.uvb) s_J += (double)jo[q];
       }
-      xH[ir] = xh;
-      r_bubble[ir] = ion ? Rtab[cross ? a.step : st] : 0.0f;
-      if (a.uvb && !ion) J21[ir] = 0.0f;
-      const int r = e / N, z = e % N;
-      const double dm = 1.0 + (double)deltax_in[(row0 + r) * (N + 2) + z];
-      s_xH += (double)xh;
-      s_xHm += (double)xh * dm;
-      s_m += dm;
-      if (a.uvb) s_J += (double)(ion ? J21[ir] : 0.0f);
+      if (a.uvb) *reinterpret_cast<float4*>(J21 + ir0) = make_float4(jo[0], jo[1], jo[2], jo[3]);
     }
-    if (zcheck && z_in[ir] < 0) {
-      z_in[ir] = (float)a.redshift;
-      if (a.uvb) J21_ion[ir] = J_21_aux * (float)a.bias;
+    if (zcheck4) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long ir = ir0 + q;
+        if (((zcheck4 >> q) & 1) && z_in[ir] < 0) {
+          z_in[ir] = (float)a.redshift;
+          if (a.uvb) J21_ion[ir] = jaux4[q] * (float)a.bias;
+        }
+      }
     }
   }
   if (a.last) {
@@ -538,6 +704,55 @@ __global__ __launch_bounds__(kNT) void k_c2r_threshold(const float2* __restrict_
       p[3] = v3;
     }
   }
+}
+
+template <int N>
+__global__ __launch_bounds__(kNT) void k_c2r_threshold(const float2* __restrict__ T, long fstride,
+                                                      const float2* __restrict__ twN,
+                                                      const float* __restrict__ deltax_in,
+                                                      uint8_t* __restrict__ state, float* __restrict__ xH,
+                                                      float* __restrict__ r_bubble, float* __restrict__ J21,
+                                                      float* __restrict__ z_in, float* __restrict__ J21_ion,
+                                                      const float* __restrict__ Rtab, double* __restrict__ partials,
+                                                      StepArgs a)
+{
+  c2r_threshold_tile<N>(T, fstride, twN, deltax_in, state, xH, r_bubble, J21, z_in, J21_ion, Rtab, partials, a);
+}
+
+// Per-box grids and scalars of a batch (device array, one entry per box).
+struct BoxDev {
+  const float* deltax;
+  float* xH;
+  float* r_bubble;
+  float* J21;
+  float* z_in;
+  float* J21_ion;
+  double eff;
+  double redshift;
+  double bias;
+};
+
+// The ladder step of every box of a cohort: blockIdx.y is the box.  T holds
+// nfield spectra per box (box stride bstride); state and partials are
+// contiguous per box; J21c_tab[box * nstep + step] is the box's J_21 constant.
+template <int N>
+__global__ __launch_bounds__(kNT) void k_c2r_threshold_batch(const float2* __restrict__ T, long fstride, long bstride,
+                                                            const float2* __restrict__ twN,
+                                                            const BoxDev* __restrict__ boxes,
+                                                            uint8_t* __restrict__ state,
+                                                            const float* __restrict__ Rtab,
+                                                            const double* __restrict__ J21c_tab, int nstep,
+                                                            double* __restrict__ partials, StepArgs a)
+{
+  const int box = blockIdx.y;
+  const BoxDev bx = boxes[box];
+  a.eff = bx.eff;
+  a.redshift = bx.redshift;
+  a.bias = bx.bias;
+  a.J21c = J21c_tab[(long)box * nstep + a.step];
+  c2r_threshold_tile<N>(T + box * bstride, fstride, twN, bx.deltax, state + (long)box * N * N * N, bx.xH,
+                        bx.r_bubble, bx.J21, bx.z_in, bx.J21_ion, Rtab,
+                        partials + (long)box * gridDim.x * 4, a);
 }
 
 // ------------------------------------------------- brightness temperature

@@ -8,6 +8,8 @@ Same names and argument meaning as the reference functions each call replaces:
 * ``filtered_field``    -> one field of the smoothing chain (find_HII_bubbles.c:214-311)
 * ``brightness_temp``   -> ComputeBrightnessTemperatureBox (BrightnessTemperature.c:24-549)
 * ``compute_ts``        -> _ComputeTs (ComputeTs.c:78-1020)
+* ``HiiBatch``          -> many boxes of _find_HII_bubbles in one graph launch
+  (no counterpart in the reference, which runs one box per call)
 
 Grids are numpy arrays (host memory, staged by the library) or torch tensors
 on the GPU (used in place, on torch's current stream).  Layouts are the
@@ -20,7 +22,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import Context, MrxError, load  # noqa: F401
-from .abi import HiiGlobals, TsGlobals, TsParams
+from .abi import HiiBatchBox, HiiGlobals, TsGlobals, TsParams
 
 try:  # torch is optional for host-memory use
     import torch
@@ -45,10 +47,21 @@ def _ptr(a):
 
 
 def _bind_stream(ctx, on_dev):
+    """Orders the library's work after torch's pending work on the current stream.
+
+    A torch side stream is handed to the library, which then enqueues on it
+    without synchronising.  torch's legacy default stream cannot be handed over
+    (the library's own stream does not wait on it), so its pending work is
+    waited for first; the library then synchronises its own stream before
+    returning.
+    """
     if on_dev:
-        ctx.check(ctx.lib.mrx_set_stream(ctx.ptr, C.c_void_p(torch.cuda.current_stream().cuda_stream)), "set_stream")
-    else:
-        ctx.check(ctx.lib.mrx_set_stream(ctx.ptr, None), "set_stream")
+        cur = torch.cuda.current_stream()
+        if cur.cuda_stream:
+            ctx.check(ctx.lib.mrx_set_stream(ctx.ptr, C.c_void_p(cur.cuda_stream)), "set_stream")
+            return
+        cur.synchronize()
+    ctx.check(ctx.lib.mrx_set_stream(ctx.ptr, None), "set_stream")
 
 
 def _empty_like_real(ref, N, fill=None):
@@ -214,3 +227,83 @@ def compute_ts(ctx, params, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_
     out = {k: getattr(g, k) for k, _ in TsGlobals._fields_}
     out.update({"x_e_box": x_e_box, "x_e_box_prev": x_e_box_prev, "Tk_box": Tk_box, "TS_box": TS_box})
     return out
+
+
+class HiiBatch:
+    """Many boxes through the bubble finder as one replayed HIP graph.
+
+    ``boxes`` is a list of dicts with ``params`` (HiiParams) and device grids
+    ``deltax``, ``stars``, ``weighted_sfr`` (padded), optional ``x_e``, and
+    optional preallocated ``xH``, ``r_bubble``, ``J_21``, ``z_at_ionization``,
+    ``J_21_at_ionization`` (allocated here otherwise, z_at_ionization = -1).
+    The grids stay bound to the plan: ``run()`` recomputes every box from the
+    current contents of its input tensors.
+    """
+
+    def __init__(self, ctx, boxes, graph=True, tune=True):
+        self.ctx = ctx
+        self.boxes = []
+        arr = (HiiBatchBox * len(boxes))()
+        for i, b in enumerate(boxes):
+            p = b["params"]
+            p.pointers_on_device = 1
+            N = p.grid_dim
+            d = b["deltax"]
+            if not _is_dev(d):
+                raise ValueError("batch grids must be torch tensors on the GPU")
+            o = {k: b.get(k) for k in ("xH", "r_bubble", "J_21", "z_at_ionization", "J_21_at_ionization")}
+            for k in ("xH", "r_bubble"):
+                if o[k] is None:
+                    o[k] = _empty_like_real(d, N)
+            if o["z_at_ionization"] is None:
+                o["z_at_ionization"] = _empty_like_real(d, N, -1.0)
+            if p.uvb_flag:
+                if o["J_21"] is None:
+                    o["J_21"] = _empty_like_real(d, N)
+                if o["J_21_at_ionization"] is None:
+                    o["J_21_at_ionization"] = _empty_like_real(d, N, 0.0)
+            keep = dict(b)
+            keep.update(o)
+            self.boxes.append(keep)
+            arr[i].params = p
+            for k in ("deltax", "stars", "weighted_sfr", "x_e", "xH", "r_bubble", "J_21", "z_at_ionization",
+                      "J_21_at_ionization"):
+                v = keep.get(k)
+                setattr(arr[i], k, None if v is None else v.data_ptr())
+        self._arr = arr
+        flags = (0 if graph else 1) | (0 if tune else 2)
+        self.ptr = C.c_void_p()
+        _bind_stream(ctx, True)
+        ctx.check(ctx.lib.mrx_hii_batch_create(ctx.ptr, arr, len(boxes), flags, C.byref(self.ptr)), "hii_batch_create")
+
+    def run(self, sync=True):
+        """Replays the plan; with ``sync`` returns the per-box globals."""
+        _bind_stream(self.ctx, True)
+        if not sync:
+            self.ctx.check(self.ctx.lib.mrx_hii_batch_run(self.ptr, None), "hii_batch_run")
+            return None
+        g = (HiiGlobals * len(self.boxes))()
+        self.ctx.check(self.ctx.lib.mrx_hii_batch_run(self.ptr, g), "hii_batch_run")
+        return [{"volume_weighted_global_xH": x.volume_weighted_global_xH,
+                 "mass_weighted_global_xH": x.mass_weighted_global_xH,
+                 "volume_weighted_global_J_21": x.volume_weighted_global_J_21, "n_radii": x.n_radii} for x in g]
+
+    def describe(self):
+        import json
+
+        buf = C.create_string_buffer(1 << 16)
+        n = self.ctx.lib.mrx_hii_batch_describe(self.ptr, buf, len(buf))
+        if n < 0:
+            raise MrxError(n, "hii_batch_describe")
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.mrx_hii_batch_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -80,3 +80,47 @@ def snapshot_inputs(N, seed, growth=1.0):
     d = gaussian_field(N, seed, growth=growth)
     stars, sfr = source_grids(d, seed)
     return to_padded(d), to_padded(stars), to_padded(sfr)
+
+
+def snapshot_inputs_torch(N, seed, device, growth=1.0):
+    """Padded (deltax, stars, weighted_sfr) generated on the GPU with torch.
+
+    The same recipe as snapshot_inputs (k^-2 field, rms 0.6, clip -0.99,
+    1e8 Msun exp(delta) Poisson(2)/2, SFR over 1e8 yr) from torch's generator,
+    so the numbers differ from the numpy stream; used where many boxes are
+    needed quickly (the batched benchmark), never as a parity input.
+    """
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    w = torch.randn((N, N, N), generator=g, device=device, dtype=torch.float32)
+    W = torch.fft.rfftn(w)
+    del w
+    kx = torch.fft.fftfreq(N, 1.0 / N, device=device)
+    kz = torch.arange(N // 2 + 1, device=device, dtype=torch.float32)
+    k2 = kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    W *= torch.rsqrt(k2)
+    W[0, 0, 0] = 0.0
+    d = torch.fft.irfftn(W, s=(N, N, N)).float()
+    del W, k2
+    d *= rms_scale(d) * float(growth)
+    d.clamp_(min=-0.99)
+    lam = torch.full((N, N, N), 2.0, device=device)
+    pois = torch.poisson(lam, generator=g)
+    mstar = 1e8 * torch.exp(d) * pois * 0.5
+    stars = mstar * float(P.HUBBLE_H / 1e10)
+    sfr_timescale = 1e8 * P.SEC_PER_YEAR / P.units()["unit_time_s"]
+    sfr = stars / float(sfr_timescale)
+
+    def pad(x):
+        out = torch.zeros((N, N, 2 * (N // 2 + 1)), device=device, dtype=torch.float32)
+        out[:, :, :N] = x
+        return out
+
+    return pad(d), pad(stars), pad(sfr)
+
+
+def rms_scale(d, rms=0.6):
+    return float(rms / float((d.double() ** 2).mean().sqrt()))

@@ -37,6 +37,7 @@ def test_struct_mirrors_match_library():
     assert L.mrx_abi_sizeof(2) == C.sizeof(abi.TbParams)
     assert L.mrx_abi_sizeof(3) == C.sizeof(abi.TsParams)
     assert L.mrx_abi_sizeof(4) == C.sizeof(abi.TsGlobals)
+    assert L.mrx_abi_sizeof(5) == C.sizeof(abi.HiiBatchBox)
 
 
 def test_radius_ladder_matches_oracle_and_closed_form():

@@ -1,0 +1,115 @@
+"""GPU: batches of boxes through the bubble finder in one graph launch.
+
+* untuned batches (default tile and radix schedules) reproduce the single-box
+  entry point bit for bit, box by box, whether replayed as a graph or enqueued
+  directly, and on every replay;
+* tuned batches (possibly other schedules, so float rounding of the FFTs
+  differs) meet the parity bar against the oracle;
+* invalid batches raise.
+"""
+import numpy as np
+import pytest
+
+from meraxes_amd import params as P
+from meraxes_amd import reion, synth
+from oracle import oracle as O
+
+from test_gpu_hii import check_parity
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def make_boxes(specs):
+    boxes = []
+    for N, L, z, ftype, eff, uvb, seed in specs:
+        d, s, w = synth.snapshot_inputs(N, seed)
+        p = P.hii_params(N, L, z, filter_type=ftype, efficiency=eff, uvb_flag=uvb)
+        boxes.append({"params": p, "host": (d, s, w), "deltax": torch.from_numpy(d).cuda(),
+                      "stars": torch.from_numpy(s).cuda(), "weighted_sfr": torch.from_numpy(w).cuda()})
+    return boxes
+
+
+SPECS = [
+    (32, 50.0, 8.0, 0, 14.0, 0, 1),
+    (32, 50.0, 8.0, 0, 33.0, 0, 2),
+    (64, 100.0, 8.0, 0, 20.0, 1, 3),
+    (64, 100.0, 9.0, 0, 11.0, 1, 4),
+    (64, 100.0, 8.5, 0, 38.0, 1, 5),
+    (128, 100.0, 8.0, 1, 25.0, 0, 6),
+]
+
+KEYS = ("xH", "r_bubble", "J_21", "z_at_ionization", "J_21_at_ionization")
+
+
+def single(ctx, b):
+    p = P.hii_params(b["params"].grid_dim, b["params"].box_size, b["params"].redshift,
+                     filter_type=b["params"].filter_type, efficiency=b["params"].efficiency,
+                     uvb_flag=b["params"].uvb_flag)
+    return reion.find_hii_bubbles(ctx, p, b["deltax"], b["stars"], b["weighted_sfr"])
+
+
+def reset_state(batch):
+    for b in batch.boxes:
+        b["z_at_ionization"].fill_(-1.0)
+        if b.get("J_21_at_ionization") is not None:
+            b["J_21_at_ionization"].fill_(0.0)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_untuned_batch_equals_single_box_calls(ctx, graph):
+    boxes = make_boxes(SPECS)
+    batch = reion.HiiBatch(ctx, boxes, graph=graph, tune=False)
+    info = batch.describe()
+    print(info, flush=True)
+    assert len(info["cohorts"]) == 4
+    assert info["graph"] == graph
+    refs = [single(ctx, b) for b in batch.boxes]
+    for rep in range(3):
+        reset_state(batch)
+        g = batch.run()
+        torch.cuda.synchronize()
+        for i, (b, r) in enumerate(zip(batch.boxes, refs)):
+            for k in KEYS:
+                if r[k] is None:
+                    continue
+                assert torch.equal(b[k], r[k]), (rep, i, k)
+            assert g[i]["volume_weighted_global_xH"] == r["volume_weighted_global_xH"]
+            assert g[i]["mass_weighted_global_xH"] == r["mass_weighted_global_xH"]
+            assert g[i]["volume_weighted_global_J_21"] == r["volume_weighted_global_J_21"]
+            assert g[i]["n_radii"] == r["n_radii"]
+    batch.close()
+
+
+def test_tuned_batch_parity_with_oracle(ctx):
+    boxes = make_boxes(SPECS)
+    batch = reion.HiiBatch(ctx, boxes, graph=True, tune=True)
+    info = batch.describe()
+    print(info, flush=True)
+    for c in info["cohorts"]:
+        assert 0 <= c["variant"] < 6
+        if c["grid_dim"] in (64, 128):
+            assert sum(1 for t in c["tune_ms"] if t > 0) >= 2  # several candidates were timed
+    g = batch.run()
+    torch.cuda.synchronize()
+    for i, b in enumerate(batch.boxes):
+        p = b["params"]
+        d, s, w = b["host"]
+        o = O.find_hii_bubbles(p, d, s, w, want_ratio=True)
+        res = {k: (b[k].cpu().numpy() if b.get(k) is not None else None) for k in KEYS}
+        res.update(g[i])
+        check_parity(res, o, p.grid_dim, f"batch box {i}", eff=p.efficiency)
+    batch.close()
+
+
+def test_batch_rejects_host_grids_and_bad_params(ctx):
+    d, s, w = synth.snapshot_inputs(32, 1)
+    p = P.hii_params(32, 50.0, 8.0)
+    with pytest.raises(ValueError):
+        reion.HiiBatch(ctx, [{"params": p, "deltax": d, "stars": s, "weighted_sfr": w}])
+    dd = [torch.from_numpy(a).cuda() for a in (d, s, w)]
+    p = P.hii_params(32, 50.0, 8.0, filter_type=9)
+    with pytest.raises(reion.MrxError) as e:
+        reion.HiiBatch(ctx, [{"params": p, "deltax": dd[0], "stars": dd[1], "weighted_sfr": dd[2]}])
+    assert e.value.code == -5
