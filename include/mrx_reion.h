@@ -115,6 +115,12 @@ typedef struct mrx_ts_params {
   const double* redshifts; /* ZZ[0..snapshot]                                            */
   double* stored_fcoll;    /* collapse-fraction history, entry [snapshot] is written      */
   const char* tables_dir;  /* directory of the 21cmFAST tables (TablesForXHeatingDir)    */
+  /* 1: the shell tables use fixed_collapse_fraction and fixed_x_e_ave instead of
+   * the two grid means the reference measures (ComputeTs.c:413-475).  The
+   * differentiable form holds them fixed (a stop-gradient on the tables). */
+  int fixed_globals;
+  double fixed_collapse_fraction;
+  double fixed_x_e_ave;
 } mrx_ts_params;
 
 /* Scalar results _ComputeTs leaves in run_globals.reion_grids (:940-985). */
@@ -233,6 +239,48 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
  */
 int mrx_profile(mrx_ctx* ctx, int enable);
 int mrx_profile_read(mrx_ctx* ctx, int max_kinds, char* names, int name_len, double* ms_total, long* counts);
+
+/* ---------------------------------------------------------------------------
+ * 21-cm power spectrum.  Drop-in for Compute_PS (ComputePowerSpectrum.c:44-255,
+ * bins from Initialise_PowerSpectrum :18-41): the field is made
+ * ((delta_T/<delta_T> - 1) V/N^3) <delta_T>, transformed, and every mode of the
+ * half spectrum adds k^3 |F|^2 / (2 pi^2 V) to its bin; bins [0, dk), [dk, f dk),
+ * [f dk, f^2 dk) ... while the ceiling stays below N dk, dk = 2 pi / box_size_mpc
+ * (box_size_mpc = BoxSize / Hubble_h), f = k_factor (1.35 when <= 1, the
+ * reference's).  delta_T is an unpadded device grid.  Writes up to max_bins
+ * entries of the bin-mean k, the power (PS_data), its error (PS_error) and
+ * the mode counts (each array nullable) and returns the number of bins.
+ *
+ * mrx_power_spectrum_adjoint: no counterpart in the reference.  For callers
+ * that differentiate a loss of the binned power: given dL/dP_b, writes
+ * dL/d(delta_T) (unpadded device grid).
+ */
+int mrx_power_spectrum(mrx_ctx* ctx, int grid_dim, double box_size_mpc, float k_factor, const float* delta_T,
+                       int max_bins, double* k, double* power, double* error, long* counts);
+int mrx_power_spectrum_adjoint(mrx_ctx* ctx, int grid_dim, double box_size_mpc, float k_factor, const float* delta_T,
+                               const double* grad_power, float* grad_delta_T);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint of one snapshot: _ComputeTs -> _find_HII_bubbles ->
+ * ComputeBrightnessTemperatureBox (ComputeTs.c:78-1020, find_HII_bubbles.c:47-495,
+ * BrightnessTemperature.c:24-124).  No counterpart in the reference; for
+ * callers that differentiate a loss of delta_T.  Given the forward's inputs
+ * (x_e_box_prev and Tk_box as they were BEFORE mrx_compute_ts updated them),
+ * its outputs (xH, TS_box, x_e_box) and dL/d(delta_T), writes dL/d(deltax),
+ * dL/d(stars) (padded), dL/d(sfr), dL/d(sfr_histories) (padded, per slot),
+ * dL/d(x_e_box_prev) (padded) and dL/d(Tk_box) (each of the last four
+ * nullable).  The ionisation mask is a stop-gradient (0 almost everywhere);
+ * cells partly ionised on the unfiltered last step carry the derivative of
+ * xH = n_f - eff f_coll.  `ts` NULL: no spin temperature (tb and hii must not
+ * use it).  With ts, the shell tables' two grid means are held at their
+ * forward values (set ts->fixed_globals for an exactly matching forward).
+ * Per-radius filtered fields are recomputed, not stored.  Device pointers only.
+ */
+int mrx_snapshot_adjoint(mrx_ctx* ctx, const mrx_ts_params* ts, const mrx_hii_params* hii, const mrx_tb_params* tb,
+                         const float* deltax, const float* stars, const float* sfr, const float* sfr_histories,
+                         const float* x_e_box_prev, const float* Tk_box, const float* xH, const float* TS_box,
+                         const float* x_e_box, const float* grad_delta_T, float* grad_deltax, float* grad_stars,
+                         float* grad_sfr, float* grad_sfr_histories, float* grad_x_e_box_prev, float* grad_Tk_box);
 
 /* ---------------------------------------------------------------------------
  * Batches of independent boxes.  No counterpart in the reference, which runs

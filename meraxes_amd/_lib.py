@@ -17,6 +17,7 @@ EXPORTS = (
     "mrx_find_hii_bubbles", "mrx_filter", "mrx_fft_r2c_3d", "mrx_fft_c2r_3d", "mrx_filtered_field",
     "mrx_brightness_temp", "mrx_abi_sizeof", "mrx_profile", "mrx_profile_read", "mrx_compute_ts",
     "mrx_hii_batch_create", "mrx_hii_batch_run", "mrx_hii_batch_describe", "mrx_hii_batch_free",
+    "mrx_power_spectrum", "mrx_power_spectrum_adjoint", "mrx_snapshot_adjoint",
 )
 
 _lib = None
@@ -58,6 +59,9 @@ def load():
     L.mrx_hii_batch_run.argtypes = [vp, C.POINTER(HiiGlobals)]
     L.mrx_hii_batch_describe.argtypes = [vp, C.c_char_p, i]
     L.mrx_hii_batch_free.argtypes = [vp]
+    L.mrx_power_spectrum.argtypes = [ctxp, i, d, f, vp, i, vp, vp, vp, vp]
+    L.mrx_power_spectrum_adjoint.argtypes = [ctxp, i, d, f, vp, vp, vp]
+    L.mrx_snapshot_adjoint.argtypes = [ctxp, vp, C.POINTER(HiiParams), C.POINTER(TbParams)] + [vp] * 16
     L.mrx_abi_sizeof.argtypes = [i]
     L.mrx_abi_sizeof.restype = C.c_size_t
     _lib = L

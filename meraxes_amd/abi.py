@@ -114,6 +114,9 @@ class TsParams(C.Structure):
         ("redshifts", C.POINTER(C.c_double)),
         ("stored_fcoll", C.POINTER(C.c_double)),
         ("tables_dir", C.c_char_p),
+        ("fixed_globals", C.c_int),
+        ("fixed_collapse_fraction", C.c_double),
+        ("fixed_x_e_ave", C.c_double),
     ]
 
 

@@ -9,6 +9,7 @@
 // codes with a message in mrx_last_error(); nothing aborts.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -19,6 +20,7 @@
 #include "../../include/mrx_reion.h"
 #include "reion_kernels.hpp"
 #include "ts_kernels.hpp"
+#include "adjoint.hpp"
 #include "ts_tables.h"
 
 using namespace mrx;
@@ -73,6 +75,12 @@ struct mrx_ctx {
   size_t ts_shells_cap = 0;
   float* wtab = nullptr;  // window table of the current radius, (N/2+1)^3 floats
   size_t wtab_cap = 0;
+  float ts_h_xe0 = 0.f, ts_h_T0 = 0.f;  // RECFAST x_e, T_K of the homogeneous branch
+  // adjoint work buffers
+  float* adj_work = nullptr;
+  size_t adj_work_cap = 0;
+  double* adj_gS = nullptr;
+  size_t adj_gS_cap = 0;
 };
 
 namespace {
@@ -248,6 +256,21 @@ int ensure_buf(mrx_ctx* ctx, float*& p, size_t& cap, size_t nfloat)
     return set_err(ctx, MRX_ENOMEM, "device allocation of %zu floats failed", nfloat);
   }
   cap = nfloat;
+  return MRX_OK;
+}
+
+template <class T>
+int ensure_dev(mrx_ctx* ctx, T*& p, size_t& cap, size_t n)
+{
+  if (cap >= n) return MRX_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  dfree(p);
+  cap = 0;
+  if (hipMalloc(&p, sizeof(T) * n) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_err(ctx, MRX_ENOMEM, "device allocation of %zu bytes failed", sizeof(T) * n);
+  }
+  cap = n;
   return MRX_OK;
 }
 
@@ -640,8 +663,11 @@ struct TsRun {
   template <int N>
   struct At {
     // grids are device pointers here: deltax, stars, sfr, hist (slots), xe_prev, Tk, TS, xe
+    // prepare_only: the shells and the snapshot constants (ctx->ts_dev) only,
+    // for the adjoint, which then runs its own per-cell kernel
     static int run(mrx_ctx* ctx, const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
-                   const float* hist, float* xe_prev, float* Tk, float* TS, mrx_ts_globals* g)
+                   const float* hist, float* xe_prev, float* Tk, float* TS, mrx_ts_globals* g,
+                   bool prepare_only = false)
     {
       using P = Pipe<N>;
       mrx_ts_data* d = ctx->tsd;
@@ -684,6 +710,9 @@ struct TsRun {
         const float T0 = (float)mrx_ts_T_RECFAST(d, zp);
         fill_ts_dev(*ctx->ts_host, d, s, p);
         HIP_TRY(hipMemcpyAsync(ctx->ts_dev, ctx->ts_host, sizeof(TsDev), hipMemcpyHostToDevice, ctx->stream));
+        ctx->ts_h_xe0 = xe0;
+        ctx->ts_h_T0 = T0;
+        if (prepare_only) return MRX_OK;
         {
           PROF("ts_homogeneous");
           k_ts_homogeneous<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, deltax, xe_prev, Tk, TS, N, xe0, T0,
@@ -714,7 +743,11 @@ struct TsRun {
           const WinArgs w = win_args(p->box_size, (float)s->R[ct], p->heating_filter_type, ct > 0 ? 1 : 0);
           MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
           MRX_TRY(P::c2r_z(ctx, ctx->T, ctx->ts_shells + (size_t)ct * ncell, 1, 3, N));
-          if (ct == 0) MRX_TRY(fcoll(s->R[0], 1));
+          if (ct == 0 && !p->fixed_globals) MRX_TRY(fcoll(s->R[0], 1));
+        }
+        if (p->fixed_globals) {
+          collapse_fraction = p->fixed_collapse_fraction;
+          x_e_ave = p->fixed_x_e_ave;
         }
         s->collapse_fraction = collapse_fraction;
         s->x_e_ave = x_e_ave;
@@ -722,6 +755,7 @@ struct TsRun {
         if (rc != MRX_OK) return set_err(ctx, rc, "spin-temperature tables failed");
         fill_ts_dev(*ctx->ts_host, d, s, p);
         HIP_TRY(hipMemcpyAsync(ctx->ts_dev, ctx->ts_host, sizeof(TsDev), hipMemcpyHostToDevice, ctx->stream));
+        if (prepare_only) return MRX_OK;
         {
           PROF("ts_cell");
           k_ts_cell<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, ctx->ts_shells, deltax, xe_prev, Tk, TS, N,
@@ -777,6 +811,211 @@ int dispatch(int N, A&&... args)
 {
   return dispatch_n<Disp<K>::template F>(N, args...);
 }
+
+// Brightness-temperature constants exactly as mrx_brightness_temp forms them
+// (BrightnessTemperature.c:46-52, :80).
+TbArgs tb_args(const mrx_tb_params* p)
+{
+  TbArgs a;
+  const float redshift = (float)(float)p->redshift;
+  const float Hubble_h = (float)(float)p->hubble_h;
+  const float OmegaM = (float)(float)p->omega_m;
+  const float OmegaB = (float)(OmegaM * p->baryon_frac);
+  a.const_factor = (float)(27.0 * (OmegaB * Hubble_h * Hubble_h / 0.023) *
+                           sqrt((0.15 / OmegaM / Hubble_h / Hubble_h) * (1.0 + redshift) / 10.0));
+  a.T_rad = kTcmb * (1. + redshift);
+  a.N = p->grid_dim;
+  a.use_ts = p->include_spin_temp;
+  return a;
+}
+
+// Bin ceilings of Compute_PS (ComputePowerSpectrum.c:18-41, :160-190), in float.
+int ps_setup(int N, double box_size_mpc, float k_factor, PsArgs& a)
+{
+  memset(&a, 0, sizeof a);
+  if (!(k_factor > 1.0f)) k_factor = 1.35f;
+  const float delta_k = (float)(2. * M_PI / box_size_mpc);
+  const float k_max = delta_k * (float)N;
+  float k_ceil = delta_k;
+  int nb = 0;
+  while (k_ceil < k_max) {
+    if (nb >= kMaxPsBins) return -1;
+    a.ceil[nb++] = k_ceil;
+    k_ceil *= k_factor;
+  }
+  a.N = N;
+  a.nbins = nb;
+  a.delta_k = delta_k;
+  a.volume = (double)powf((float)box_size_mpc, 3);
+  return nb;
+}
+
+__global__ void k_sum_f32(const float* __restrict__ x, long n, double* __restrict__ partials)
+{
+  __shared__ double red[kNT];
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) acc += x[i];
+  acc = block_sum<kNT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+struct PowerSpec {
+  template <int N>
+  struct At {
+    // forward (grad_power == NULL): bins into k/power/error/counts; adjoint:
+    // dL/d(delta_T) into grad_out from dL/dP_b
+    static int run(mrx_ctx* ctx, double box_size_mpc, float k_factor, const float* dT, int max_bins, double* k_out,
+                   double* p_out, double* e_out, long* c_out, const double* grad_power, float* grad_out)
+    {
+      using P = Pipe<N>;
+      PsArgs a;
+      const int nb = ps_setup(N, box_size_mpc, k_factor, a);
+      if (nb < 1) return set_err(ctx, MRX_EINVAL, "power spectrum: bad binning (%d bins)", nb);
+      const long ncell = (long)N * N * N;
+      const size_t np = npad(N);
+      const long nmodes = (long)N * N * (N / 2 + 1);
+      const int nblk = (int)((nmodes + (long)kNT * kPsJ - 1) / ((long)kNT * kPsJ));
+      MRX_TRY(ensure_buffers(ctx, N, 1));
+      MRX_TRY(ensure_buf(ctx, ctx->adj_work, ctx->adj_work_cap, 2 * np));
+      // partials: block sums, then the bin totals, then dL/dP_b / n_b
+      const size_t pblk = std::max((size_t)nblk * nb * 3, (size_t)1024);
+      MRX_TRY(ensure_partials(ctx, pblk + 4 * kMaxPsBins));
+      if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
+      double* sums_big = ctx->partials + pblk;
+      double* dlam = sums_big + 3 * kMaxPsBins;
+      // <delta_T>
+      k_sum_f32<<<1024, kNT, 0, ctx->stream>>>(dT, ncell, ctx->partials);
+      MRX_TRY(check_launch(ctx, "k_sum_f32"));
+      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, 1024, 1, ctx->sums);
+      MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+      double tot = 0.0;
+      HIP_TRY(hipMemcpyAsync(&tot, ctx->sums, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      const double ave = tot / pow((double)N, 3);
+      float* work = ctx->adj_work;
+      k_ps_prep<<<(unsigned)((ncell + 255) / 256), 256, 0, ctx->stream>>>(dT, work, N, ave, (float)a.volume);
+      MRX_TRY(check_launch(ctx, "k_ps_prep"));
+      MRX_TRY(P::forward(ctx, work, 0, 1, false, ctx->U, ctx->V));
+      k_ps_bin<<<nblk, kNT, 0, ctx->stream>>>(ctx->U, ctx->V, ctx->partials, a);
+      MRX_TRY(check_launch(ctx, "k_ps_bin"));
+      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nblk, nb * 3, sums_big);
+      MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+      std::vector<double> h(3 * (size_t)nb);
+      HIP_TRY(hipMemcpyAsync(h.data(), sums_big, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      if (!grad_power) {
+        for (int b = 0; b < nb && b < max_bins; ++b) {
+          const double cnt = h[3 * b + 2];
+          if (k_out) k_out[b] = h[3 * b + 1] / cnt;
+          if (p_out) p_out[b] = h[3 * b] / cnt;
+          if (e_out) e_out[b] = (h[3 * b] / cnt) / sqrt(cnt);
+          if (c_out) c_out[b] = (long)cnt;
+        }
+        return nb;
+      }
+      std::vector<double> lam(nb);
+      for (int b = 0; b < nb; ++b) lam[b] = (h[3 * b + 2] > 0) ? grad_power[b] / h[3 * b + 2] : 0.0;
+      HIP_TRY(hipMemcpyAsync(dlam, lam.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+      k_ps_adjoint_scale<<<(unsigned)((nmodes + 255) / 256), 256, 0, ctx->stream>>>(
+        ctx->U, ctx->V, dlam, a, a.volume / pow((double)N, 3));
+      MRX_TRY(check_launch(ctx, "k_ps_adjoint_scale"));
+      const WinArgs w = win_args(box_size_mpc, 1.0f, 0, 0);
+      MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
+      MRX_TRY(P::c2r_z(ctx, ctx->T, grad_out, 1, -1, N));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      return nb;
+    }
+  };
+};
+
+struct SnapAdj {
+  template <int N>
+  struct At {
+    static int run(mrx_ctx* ctx, const mrx_ts_params* tp, const mrx_hii_params* hp, const mrx_tb_params* bp,
+                   const float* deltax, const float* stars, const float* sfr, const float* hist,
+                   const float* xe_prev, const float* Tk, const float* xH, const float* TS, const float* xe_box,
+                   const float* gdT, float* g_delta, float* g_stars, float* g_sfr, float* g_hist, float* g_xe_prev,
+                   float* g_Tk)
+    {
+      using P = Pipe<N>;
+      const long ncell = (long)N * N * N;
+      const size_t np = npad(N);
+      AdjCellArgs a;
+      memset(&a, 0, sizeof a);
+      a.N = N;
+      const TbArgs tb = tb_args(bp);
+      a.tb_const = tb.const_factor;
+      a.T_rad = tb.T_rad;
+      a.use_ts = tb.use_ts;
+      std::vector<double> R;
+      const int S = ladder(hp, R);
+      if (S < 1) return set_err(ctx, MRX_EINVAL, "radius ladder is empty");
+      const double Rl = R[S - 1];
+      const double pixel_volume = pow(hp->box_size / (double)N, 3);
+      a.eff = hp->efficiency;
+      a.fc_unit = (4.0 / 3.0) * M_PI * (Rl * Rl * Rl) / pixel_volume / RtoM(hp, Rl);
+      a.use_xe = hp->include_spin_temp;
+      const mrx_ts_snap* s = nullptr;
+      if (tp) {
+        MRX_TRY(TsRun::At<N>::run(ctx, tp, deltax, stars, sfr, hist, const_cast<float*>(xe_prev), nullptr, nullptr,
+                                  nullptr, true));
+        s = ctx->tss;
+        a.ts_mode = s->homogeneous ? 1 : 2;
+        a.h_xe0 = ctx->ts_h_xe0;
+        a.h_T0 = ctx->ts_h_T0;
+      }
+      const bool shells = (a.ts_mode == 2) && !s->no_light;
+      if (a.ts_mode == 2) MRX_TRY(ensure_dev(ctx, ctx->adj_gS, ctx->adj_gS_cap, 4 * (size_t)ncell));
+      const unsigned nbc = (unsigned)((ncell + kNT - 1) / kNT);
+      {
+        PROF("adjoint_cell");
+        k_snapshot_adjoint_cell<<<nbc, kNT, 0, ctx->stream>>>(
+          ctx->ts_dev, ctx->ts_shells, deltax, stars, xe_prev, Tk, xH, TS, xe_box, gdT, g_delta, g_stars,
+          a.ts_mode == 2 ? g_xe_prev : nullptr, a.ts_mode == 2 ? g_Tk : nullptr, ctx->adj_gS, a);
+      }
+      MRX_TRY(check_launch(ctx, "k_snapshot_adjoint_cell"));
+      if (a.ts_mode != 2) {
+        if (g_xe_prev) HIP_TRY(hipMemsetAsync(g_xe_prev, 0, sizeof(float) * np, ctx->stream));
+        if (g_Tk) HIP_TRY(hipMemsetAsync(g_Tk, 0, sizeof(float) * ncell, ctx->stream));
+      }
+      const int nslot = (tp && s && !s->homogeneous) ? tp->n_history : 0;
+      if (g_sfr) HIP_TRY(hipMemsetAsync(g_sfr, 0, sizeof(float) * np, ctx->stream));
+      if (g_hist && tp) HIP_TRY(hipMemsetAsync(g_hist, 0, sizeof(float) * np * (size_t)tp->n_history, ctx->stream));
+      if (!shells || !g_sfr) return MRX_OK;
+      MRX_TRY(ensure_buf(ctx, ctx->adj_work, ctx->adj_work_cap, 2 * np));
+      float* gshell = ctx->adj_work;
+      float* tmp = ctx->adj_work + np;
+      const long half = (long)N * N * (N / 2 + 1);
+      for (int ct = s->nsteps - 1; ct >= 0; --ct) {
+        {
+          PROF("adjoint_shell_grad");
+          k_ts_shell_grad<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, ctx->ts_shells, xe_prev, ctx->adj_gS, gshell, ct,
+                                                        N);
+        }
+        MRX_TRY(check_launch(ctx, "k_ts_shell_grad"));
+        const float* add = gshell;
+        if (ct > 0) {  // shell 0 is the unfiltered grid
+          MRX_TRY(P::forward(ctx, gshell, 0, 1, true, ctx->U, ctx->V, true));
+          const WinArgs w = win_args(tp->box_size, (float)s->R[ct], tp->heating_filter_type, 1);
+          MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
+          MRX_TRY(P::c2r_z(ctx, ctx->T, tmp, 1, -1, N + 2));
+          add = tmp;
+        }
+        k_axpy<<<(unsigned)((np + 255) / 256), 256, 0, ctx->stream>>>(g_sfr, add, (long)np);
+        MRX_TRY(check_launch(ctx, "k_axpy"));
+        for (int k = s->n_ops[ct] - 1; k >= 0; --k) {
+          const mrx_sfr_op& o = s->ops[ct][k];
+          const long n = (o.kind <= 1) ? half : ncell;
+          float* gh = (o.kind >= 2 && g_hist && o.slot < nslot) ? g_hist + (size_t)o.slot * np : nullptr;
+          k_sfr_op_adjoint<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(g_sfr, gh, o.kind, o.weight,
+                                                                                 (float)o.weight, N, half);
+          MRX_TRY(check_launch(ctx, "k_sfr_op_adjoint"));
+        }
+      }
+      return MRX_OK;
+    }
+  };
+};
 
 #include "batch.hpp"
 
@@ -884,6 +1123,8 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->ts_work);
   dfree(ctx->ts_shells);
   dfree(ctx->wtab);
+  dfree(ctx->adj_work);
+  dfree(ctx->adj_gS);
   delete ctx->tsd;
   delete ctx->tss;
   delete ctx->ts_host;
@@ -1230,6 +1471,78 @@ int mrx_profile_read(mrx_ctx* ctx, int max_kinds, char* names, int name_len, dou
     counts[k] = cnt[k];
   }
   return n;
+}
+
+int mrx_power_spectrum(mrx_ctx* ctx, int grid_dim, double box_size_mpc, float k_factor, const float* delta_T,
+                       int max_bins, double* k, double* power, double* error, long* counts)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!supported_n(grid_dim)) return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not supported", grid_dim);
+  if (!delta_T || !(box_size_mpc > 0) || max_bins < 0) return set_err(ctx, MRX_EINVAL, "bad power spectrum arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  return dispatch<PowerSpec>(grid_dim, ctx, box_size_mpc, k_factor, delta_T, max_bins, k, power, error, counts,
+                             (const double*)nullptr, (float*)nullptr);
+}
+
+int mrx_power_spectrum_adjoint(mrx_ctx* ctx, int grid_dim, double box_size_mpc, float k_factor, const float* delta_T,
+                               const double* grad_power, float* grad_delta_T)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!supported_n(grid_dim)) return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not supported", grid_dim);
+  if (!delta_T || !grad_power || !grad_delta_T || !(box_size_mpc > 0))
+    return set_err(ctx, MRX_EINVAL, "bad power spectrum adjoint arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int rc = dispatch<PowerSpec>(grid_dim, ctx, box_size_mpc, k_factor, delta_T, 0, (double*)nullptr,
+                                     (double*)nullptr, (double*)nullptr, (long*)nullptr, grad_power, grad_delta_T);
+  return rc < 0 ? rc : MRX_OK;
+}
+
+int mrx_snapshot_adjoint(mrx_ctx* ctx, const mrx_ts_params* ts, const mrx_hii_params* hii, const mrx_tb_params* tb,
+                         const float* deltax, const float* stars, const float* sfr, const float* sfr_histories,
+                         const float* x_e_box_prev, const float* Tk_box, const float* xH, const float* TS_box,
+                         const float* x_e_box, const float* grad_delta_T, float* grad_deltax, float* grad_stars,
+                         float* grad_sfr, float* grad_sfr_histories, float* grad_x_e_box_prev, float* grad_Tk_box)
+{
+  if (!ctx) return MRX_EINVAL;
+  MRX_TRY(validate_hii(ctx, hii));
+  if (!tb || tb->grid_dim != hii->grid_dim) return set_err(ctx, MRX_EINVAL, "adjoint: tb params missing or mismatched");
+  if (!hii->pointers_on_device || !tb->pointers_on_device || (ts && !ts->pointers_on_device))
+    return set_err(ctx, MRX_EINVAL, "adjoint: grids must be device pointers");
+  if (!deltax || !stars || !xH || !grad_delta_T || !grad_deltax || !grad_stars)
+    return set_err(ctx, MRX_EINVAL, "adjoint: a required grid pointer is NULL");
+  if (tb->include_spin_temp && !TS_box) return set_err(ctx, MRX_EINVAL, "adjoint: include_spin_temp needs TS_box");
+  if (hii->include_spin_temp && !x_e_box) return set_err(ctx, MRX_EINVAL, "adjoint: include_spin_temp needs x_e_box");
+  if (ts) {
+    if (ts->grid_dim != hii->grid_dim) return set_err(ctx, MRX_EINVAL, "adjoint: ts grid_dim mismatch");
+    if (!sfr || !x_e_box_prev || !Tk_box || !TS_box || !ts->redshifts || !ts->stored_fcoll || !ts->tables_dir)
+      return set_err(ctx, MRX_EINVAL, "adjoint: spin-temperature inputs missing");
+    if (ts->num_filter_steps < 1 || ts->num_filter_steps > kTsMaxSteps)
+      return set_err(ctx, MRX_EUNSUPPORTED, "TsNumFilterSteps %d outside 1..%d", ts->num_filter_steps, kTsMaxSteps);
+    if (ts->heating_filter_type < 0 || ts->heating_filter_type > 2)
+      return set_err(ctx, MRX_EFILTER, "TsHeatingFilterType %d is undefined", ts->heating_filter_type);
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ts) {
+    const std::string key = ts_key(ts);
+    if (!ctx->tsd || ctx->ts_key != key) {
+      if (!ctx->tsd) ctx->tsd = new mrx_ts_data();
+      if (mrx_ts_data_init(ctx->tsd, ts) != MRX_OK) {
+        ctx->ts_key.clear();
+        return set_err(ctx, MRX_EINVAL, "cannot read the 21cmFAST tables under %s", ts->tables_dir);
+      }
+      ctx->ts_key = key;
+    }
+    if (!ctx->tss) ctx->tss = new mrx_ts_snap();
+    if (!ctx->ts_host) ctx->ts_host = new TsDev();
+    if (mrx_ts_plan(ctx->tss, ctx->tsd, ts) != MRX_OK) return set_err(ctx, MRX_EINVAL, "bad spin-temperature plan");
+    const mrx_ts_snap* s = ctx->tss;
+    if (!s->homogeneous && s->max_slot >= 0 && (!sfr_histories || s->max_slot >= ts->n_history))
+      return set_err(ctx, MRX_EINVAL, "the shells need SFR history slot %d, %d held", s->max_slot, ts->n_history);
+  }
+  MRX_TRY(dispatch<SnapAdj>(hii->grid_dim, ctx, ts, hii, tb, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box,
+                            xH, TS_box, x_e_box, grad_delta_T, grad_deltax, grad_stars, grad_sfr, grad_sfr_histories,
+                            grad_x_e_box_prev, grad_Tk_box));
+  return finish(ctx);
 }
 
 int mrx_hii_batch_create(mrx_ctx* ctx, const mrx_hii_batch_box* boxes, int n_boxes, int flags,

@@ -60,33 +60,42 @@ struct TsDev {
 
 #pragma clang fp contract(off)
 
-__device__ __forceinline__ double spline_eval(const KappaSpline& s, double xv)
+// The scalar helpers below are templates over the arithmetic type R: double
+// for the forward kernels, a forward-mode dual number for the adjoint, which
+// then recomputes exactly the forward's primal values.  pval() is the primal
+// value, used for every branch.
+__device__ __forceinline__ double pval(double x) { return x; }
+
+template <class R>
+__device__ __forceinline__ R spline_eval(const KappaSpline& s, R xv)
 {
   int lo = 0, hi = s.n - 1;
-  if (xv <= s.x[0]) {
+  const double xp = pval(xv);
+  if (xp <= s.x[0]) {
     lo = 0;
-  } else if (xv >= s.x[hi]) {
+  } else if (xp >= s.x[hi]) {
     lo = hi - 1;
   } else {
     while (hi - lo > 1) {
       const int mid = (lo + hi) / 2;
-      if (s.x[mid] > xv)
+      if (s.x[mid] > xp)
         hi = mid;
       else
         lo = mid;
     }
   }
-  const double dx = xv - s.x[lo];
+  const R dx = xv - s.x[lo];
   return s.y[lo] + dx * (s.b[lo] + dx * (s.c[lo] + dx * s.d[lo]));
 }
 
 // kappa_10 (H-H) with its below-table clamp and T^0.381 extrapolation (XRayHeatingFunctions.c:1684-1693)
-__device__ __forceinline__ double kappa_HH(const KappaSpline& s, double TK)
+template <class R>
+__device__ __forceinline__ R kappa_HH(const KappaSpline& s, R TK)
 {
-  double ans;
-  if (log(TK) < s.x[0])
-    ans = s.y[0];
-  else if (log(TK) > s.x[s.n - 1])
+  R ans;
+  if (pval(log(TK)) < s.x[0])
+    ans = R(s.y[0]);
+  else if (pval(log(TK)) > s.x[s.n - 1])
     ans = log(exp(s.y[s.n - 1]) * pow(TK / exp(s.x[s.n - 1]), 0.381));
   else
     ans = spline_eval(s, log(TK));
@@ -94,68 +103,82 @@ __device__ __forceinline__ double kappa_HH(const KappaSpline& s, double TK)
 }
 
 // kappa_10_elec / kappa_10_pH with linear-in-log extrapolation (:1754-1766, :1827-1839)
-__device__ __forceinline__ double kappa_file(const KappaSpline& s, double T)
+template <class R>
+__device__ __forceinline__ R kappa_file(const KappaSpline& s, R T)
 {
   const int n = s.n;
-  double ans;
+  R ans;
   T = log(T);
-  if (T < s.x[0])
-    ans = s.y[0];
-  else if (T > s.x[n - 1])
+  if (pval(T) < s.x[0])
+    ans = R(s.y[0]);
+  else if (pval(T) > s.x[n - 1])
     ans = s.y[n - 1] + ((s.y[n - 1] - s.y[n - 2]) / (s.x[n - 1] - s.x[n - 2]) * (T - s.x[n - 1]));
   else
     ans = spline_eval(s, T);
   return exp(ans);
 }
 
-// get_Ts (XRayHeatingFunctions.c:1513-1558) at this snapshot's z; float in, float out
+// get_Ts (XRayHeatingFunctions.c:1513-1558) at this snapshot's z, on the
+// float inputs widened to R; the caller rounds the result to float.
+template <class R>
+__device__ __forceinline__ R get_Ts_t(const TsDev& t, R dl, R TK, R xe, R Jalpha, R* xa_out)
+{
+  const double Trad = t.Trad;
+  // xcoll = xcoll_HI + xcoll_elec + xcoll_prot (:1560-1600)
+  const R nH = (1.0 - xe) * t.No * t.P3 * (1.0 + dl);
+  const R xHI = t.T21_Trad * nH * kappa_HH(t.kHH, TK) / 2.85e-15;
+  const R ne = xe * t.N_b0 * t.P3 * (1.0 + dl);
+  const R xel = t.T21_Trad * ne * kappa_file(t.keH, TK) / 2.85e-15;
+  const R np = xe * t.No * t.P3 * (1.0 + dl);
+  const R xpr = t.T21_Trad * np * kappa_file(t.kpH, TK) / 2.85e-15;
+  const R xc = xHI + xel + xpr;
+  R TS;
+  if (pval(Jalpha) > 1.0e-20) {
+    R TSold(0.0), xa(0.0);
+    TS = R(Trad);
+    const R tgp = t.TGP * (1.0 + dl) * (1.0 - xe);
+    while (fabs(pval(TS) - pval(TSold)) / pval(TS) > 1.0e-3) {
+      TSold = TS;
+      const R xi = pow(1.0e-7 * tgp / TK / TK, 1.0 / 3.0);
+      R S = 1.0 - 0.0631789 / TK + 0.115995 / TK / TK - 0.401403 / TS / TK;
+      S += 0.336463 / TS / TK / TK;
+      S /= 1.0 + 2.98394 * xi + 1.53583 * xi * xi + 3.85289 * xi * xi * xi;
+      xa = t.XA * S * Jalpha;
+      const R ce = 1.0 / TK + 0.405535 / TK * (1.0 / TS - 1.0 / TK);
+      const R Tceff = 1.0 / ce;
+      const R TSinv = (1.0 / Trad + xa / Tceff + xc / TK) / (1.0 + xa + xc);
+      TS = 1.0 / TSinv;
+    }
+    *xa_out = xa;
+  } else {
+    const R TSinv = (1.0 / Trad + xc / TK) / (1.0 + xc);
+    TS = 1.0 / TSinv;
+    *xa_out = R(0.0);
+  }
+  if (pval(TS) < 0.) TS = fabs(TS);
+  return TS;
+}
+
+// float in, float out, as the reference calls it
 __device__ __forceinline__ float dev_get_Ts(const TsDev& t, float delta, float TK_f, float xe_f, float Jalpha,
                                            float* xa_out)
 {
-  const double TK = TK_f, xe = xe_f, dl = delta;
-  const double Trad = t.Trad;
-  // xcoll = xcoll_HI + xcoll_elec + xcoll_prot (:1560-1600)
-  const double nH = (1.0 - xe) * t.No * t.P3 * (1.0 + dl);
-  const double xHI = t.T21_Trad * nH * kappa_HH(t.kHH, TK) / 2.85e-15;
-  const double ne = xe * t.N_b0 * t.P3 * (1.0 + dl);
-  const double xel = t.T21_Trad * ne * kappa_file(t.keH, TK) / 2.85e-15;
-  const double np = xe * t.No * t.P3 * (1.0 + dl);
-  const double xpr = t.T21_Trad * np * kappa_file(t.kpH, TK) / 2.85e-15;
-  const double xc = xHI + xel + xpr;
-  double TS;
-  if (Jalpha > 1.0e-20) {
-    double TSold = 0.0, xa = 0.0;
-    TS = Trad;
-    const double tgp = t.TGP * (1.0 + dl) * (1.0 - xe);
-    while (fabs(TS - TSold) / TS > 1.0e-3) {
-      TSold = TS;
-      const double xi = pow(1.0e-7 * tgp / TK / TK, 1.0 / 3.0);
-      double S = 1.0 - 0.0631789 / TK + 0.115995 / TK / TK - 0.401403 / TS / TK;
-      S += 0.336463 / TS / TK / TK;
-      S /= 1.0 + 2.98394 * xi + 1.53583 * xi * xi + 3.85289 * xi * xi * xi;
-      xa = t.XA * S * (double)Jalpha;
-      const double ce = 1.0 / TK + 0.405535 / TK * (1.0 / TS - 1.0 / TK);
-      const double Tceff = 1.0 / ce;
-      const double TSinv = (1.0 / Trad + xa / Tceff + xc / TK) / (1.0 + xa + xc);
-      TS = 1.0 / TSinv;
-    }
-    *xa_out = (float)xa;
-  } else {
-    const double TSinv = (1.0 / Trad + xc / TK) / (1.0 + xc);
-    TS = 1.0 / TSinv;
-    *xa_out = 0.0f;
-  }
-  if (TS < 0.) TS = fabs(TS);
+  double xa;
+  const double TS = get_Ts_t<double>(t, (double)delta, (double)TK_f, (double)xe_f, (double)Jalpha, &xa);
+  *xa_out = (float)xa;
   return (float)TS;
 }
 
-__device__ __forceinline__ double dev_alpha_A(double T)
+template <class R>
+__device__ __forceinline__ R alpha_A_t(R T)
 {
-  const double logT = log(T / (double)1.1604505e4);
+  const R logT = log(T / (double)1.1604505e4);
   return exp(-28.6130338 - 0.72411256 * logT - 2.02604473e-2 * pow(logT, 2) - 2.38086188e-3 * pow(logT, 3) -
              3.21260521e-4 * pow(logT, 4) - 1.42150291e-5 * pow(logT, 5) + 4.98910892e-6 * pow(logT, 6) +
              5.75561414e-7 * pow(logT, 7) - 1.85676704e-8 * pow(logT, 8) - 3.07113524e-9 * pow(logT, 9));
 }
+
+__device__ __forceinline__ double dev_alpha_A(double T) { return alpha_A_t<double>(T); }
 
 // Per-cell spin-temperature update (ComputeTs.c:743-900); 7 block partial sums:
 // J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e.

@@ -8,8 +8,12 @@ Same names and argument meaning as the reference functions each call replaces:
 * ``filtered_field``    -> one field of the smoothing chain (find_HII_bubbles.c:214-311)
 * ``brightness_temp``   -> ComputeBrightnessTemperatureBox (BrightnessTemperature.c:24-549)
 * ``compute_ts``        -> _ComputeTs (ComputeTs.c:78-1020)
+* ``power_spectrum``    -> Compute_PS (ComputePowerSpectrum.c:44-255)
 * ``HiiBatch``          -> many boxes of _find_HII_bubbles in one graph launch
   (no counterpart in the reference, which runs one box per call)
+* ``power_spectrum_adjoint``, ``snapshot_adjoint`` -> backward passes of
+  Compute_PS and of one snapshot (Ts -> bubbles -> delta_T), no counterpart
+  in the reference; meraxes_amd.autograd wraps them as autograd functions
 
 Grids are numpy arrays (host memory, staged by the library) or torch tensors
 on the GPU (used in place, on torch's current stream).  Layouts are the
@@ -227,6 +231,69 @@ def compute_ts(ctx, params, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_
     out = {k: getattr(g, k) for k, _ in TsGlobals._fields_}
     out.update({"x_e_box": x_e_box, "x_e_box_prev": x_e_box_prev, "Tk_box": Tk_box, "TS_box": TS_box})
     return out
+
+
+def power_spectrum(ctx, delta_T, box_size_mpc, k_factor=1.35, max_bins=64):
+    """Binned 21-cm power of an unpadded device grid: dict of k, power, error, counts."""
+    N = delta_T.shape[0]
+    if not _is_dev(delta_T):
+        raise ValueError("power_spectrum takes a device grid")
+    k = np.zeros(max_bins)
+    pw = np.zeros(max_bins)
+    er = np.zeros(max_bins)
+    cn = np.zeros(max_bins, np.int64)
+    _bind_stream(ctx, True)
+    n = ctx.lib.mrx_power_spectrum(ctx.ptr, N, float(box_size_mpc), float(k_factor), _ptr(delta_T.contiguous()),
+                                   max_bins, k.ctypes.data_as(C.c_void_p), pw.ctypes.data_as(C.c_void_p),
+                                   er.ctypes.data_as(C.c_void_p), cn.ctypes.data_as(C.c_void_p))
+    ctx.check(0 if n >= 0 else n, "power_spectrum")
+    return {"k": k[:n].copy(), "power": pw[:n].copy(), "error": er[:n].copy(), "counts": cn[:n].copy()}
+
+
+def power_spectrum_adjoint(ctx, delta_T, box_size_mpc, grad_power, k_factor=1.35, out=None):
+    """dL/d(delta_T) from dL/dP_b (a host array over the bins)."""
+    N = delta_T.shape[0]
+    g = np.ascontiguousarray(np.asarray(grad_power, np.float64))
+    if out is None:
+        out = torch.empty((N, N, N), dtype=torch.float32, device=delta_T.device)
+    _bind_stream(ctx, True)
+    ctx.check(ctx.lib.mrx_power_spectrum_adjoint(ctx.ptr, N, float(box_size_mpc), float(k_factor),
+                                                 _ptr(delta_T.contiguous()), g.ctypes.data_as(C.c_void_p),
+                                                 _ptr(out)), "power_spectrum_adjoint")
+    return out
+
+
+def snapshot_adjoint(ctx, ts_params, hii_params, tb_params, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box,
+                     xH, TS_box, x_e_box, grad_delta_T, want=("deltax", "stars", "sfr", "sfr_histories",
+                                                             "x_e_box_prev", "Tk_box")):
+    """Backward pass of one snapshot; returns {name: gradient} for ``want``.
+
+    x_e_box_prev / Tk_box are the values BEFORE the forward's spin-temperature
+    step; xH, TS_box, x_e_box are the forward's outputs.
+    """
+    for p in (ts_params, hii_params, tb_params):
+        if p is not None:
+            p.pointers_on_device = 1
+    g = {}
+    g["deltax"] = torch.empty_like(deltax)
+    g["stars"] = torch.empty_like(stars)
+    if ts_params is not None:
+        if "sfr" in want:
+            g["sfr"] = torch.empty_like(sfr)
+        if "sfr_histories" in want and sfr_histories is not None:
+            g["sfr_histories"] = torch.empty_like(sfr_histories)
+        if "x_e_box_prev" in want:
+            g["x_e_box_prev"] = torch.empty_like(x_e_box_prev)
+        if "Tk_box" in want:
+            g["Tk_box"] = torch.empty_like(Tk_box)
+    _bind_stream(ctx, True)
+    ctx.check(ctx.lib.mrx_snapshot_adjoint(
+        ctx.ptr, None if ts_params is None else C.byref(ts_params), C.byref(hii_params), C.byref(tb_params),
+        _ptr(deltax), _ptr(stars), _ptr(sfr), _ptr(sfr_histories), _ptr(x_e_box_prev), _ptr(Tk_box), _ptr(xH),
+        _ptr(TS_box), _ptr(x_e_box), _ptr(grad_delta_T.contiguous()), _ptr(g["deltax"]), _ptr(g["stars"]),
+        _ptr(g.get("sfr")), _ptr(g.get("sfr_histories")), _ptr(g.get("x_e_box_prev")), _ptr(g.get("Tk_box"))),
+        "snapshot_adjoint")
+    return g
 
 
 class HiiBatch:

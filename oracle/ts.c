@@ -285,6 +285,10 @@ int orc_compute_ts(const mrx_ts_params* p, const float* deltax, const float* sta
         x_e_ave /= total_n_cells;
       }
     }
+    if (p->fixed_globals) {
+      collapse_fraction = p->fixed_collapse_fraction;
+      x_e_ave = p->fixed_x_e_ave;
+    }
     s->collapse_fraction = collapse_fraction;
     s->x_e_ave = x_e_ave;
     mrx_ts_build_tables(s, d, p);

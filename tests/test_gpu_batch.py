@@ -63,7 +63,8 @@ def test_untuned_batch_equals_single_box_calls(ctx, graph):
     batch = reion.HiiBatch(ctx, boxes, graph=graph, tune=False)
     info = batch.describe()
     print(info, flush=True)
-    assert len(info["cohorts"]) == 4
+    # the three 64^3 boxes differ only in z and efficiency: one cohort
+    assert len(info["cohorts"]) == 3
     assert info["graph"] == graph
     refs = [single(ctx, b) for b in batch.boxes]
     for rep in range(3):

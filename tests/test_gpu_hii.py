@@ -89,9 +89,13 @@ def test_uvb_and_carried_state(ctx):
     flips = check_parity(g, o, N, "UVB", eff=60.0)
     keep = ~flips
     both = keep & (o["xH"] < 1e-5)
-    rel = np.abs(g["J_21"][both] - o["J_21"][both]) / np.maximum(np.abs(o["J_21"][both]), 1e-30)
-    print(f"J_21 rel diff max {rel.max() if rel.size else 0:.2e}", flush=True)
-    assert (rel <= 1e-4).mean() > 0.999
+    # J_21 is the filtered SFR at the first-crossing radius: float transform
+    # error relative to the field's scale, large relative to small values
+    dJ = np.abs(g["J_21"][both].astype(np.float64) - o["J_21"][both])
+    scale = float(np.sqrt(np.mean(o["J_21"][both].astype(np.float64) ** 2)))
+    rel = dJ / np.maximum(np.abs(o["J_21"][both]), 1e-30)
+    print(f"J_21 rel diff max {rel.max() if rel.size else 0:.2e}, max |d|/rms {dJ.max() / scale:.2e}", flush=True)
+    assert (dJ <= 1e-4 * np.abs(o["J_21"][both]) + 1e-6 * scale).mean() > 0.999
     assert abs(g["volume_weighted_global_J_21"] - o["volume_weighted_global_J_21"]) <= 1e-4 * abs(
         o["volume_weighted_global_J_21"]) + 1e-30
     carried = z_in > 0
