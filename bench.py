@@ -44,13 +44,23 @@ def model_bytes_hii(N, F, S, uvb):
 
 
 def kernel_bytes(name, N, F):
-    """Algorithmic HBM bytes of one launch of a kernel (packed spectra: 8*N*N*N/2 bytes)."""
+    """Algorithmic HBM bytes of one launch of a kernel (packed spectra: 8*N*N*N/2 bytes).
+
+    Names with a ts_ prefix are launches of the spin-temperature shell loop,
+    one field each; the others are the bubble finder's, F fields each.
+    """
+    if name.startswith("ts_"):
+        name, F = name[3:], 1
+        if name == "c2r_z":  # shell: spectrum in, float shell out
+            return 8 * N * N * (N // 2) + 4 * N**3
+        if name == "cell":
+            name = "ts_cell"
     spec = 8 * N * N * (N // 2)
     Nr = N**3
     table = {
         # strided pencil passes: read + write one spectrum per field
         "colpass_y_inv": 2 * F * spec,
-        "colpass_y_fwd": 2 * spec,
+        "colpass_y_fwd": 2 * spec,  # forward passes run one field per launch
         "ts_cell": 40 * 4 * Nr + 4 * N * N * (N + 2) * 3 + 3 * 4 * Nr,
         "colpass_x_fwd": 2 * spec,
         # window + inverse x pass: read unfiltered (+ Nyquist plane) and write per field
@@ -58,6 +68,7 @@ def kernel_bytes(name, N, F):
         # inverse z pass + threshold: read each field's spectrum, 1 byte of step state per cell
         "c2r_threshold": F * spec + 2 * Nr,
         "r2c_z": 4 * N * N * (N + 2) + spec,
+        "window_table": 4 * (N // 2 + 1) ** 3,
         "brightness_temp": 4 * N * N * (N + 2) + 2 * 4 * Nr,
     }
     return table.get(name)
@@ -116,6 +127,90 @@ def cpu_baseline(snapshot):
         "kind": "port",
         "sample": f"oracle ref32: {what}; {dt:.1f} s wall",
     }
+
+
+def measured_traffic(kernel, N):
+    """HBM bytes per launch of `kernel` from the committed PMC summary, or None.
+
+    profiles/traffic_<kernel>_<N>.json holds FETCH_SIZE + WRITE_SIZE per
+    dispatch from rocprofv3 --pmc, scaled by the same counters' reading of a
+    copy kernel of known size (see the summary beside it).
+    """
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{N}.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def bench_adjoint(ctx, N, steps=2):
+    """Forward (Ts + bubbles + dTb + power-spectrum loss) vs its hand-written backward at N^3.
+
+    The loss is the squared difference of the binned 21-cm power (20 bins) to
+    that of a second seeded realisation; gradients reach deltax and stars.
+    """
+    import numpy as np
+    import torch
+
+    from meraxes_amd import autograd as MA
+    from meraxes_amd import params as P
+    from meraxes_amd import synth
+
+    dev = torch.device("cuda:0")
+    L = 400.0 * N / 512
+    zz = P.snapshot_redshifts()
+    snap = 24
+    z = float(zz[snap])
+    nh = 8
+    log(f"[bench] adjoint at {N}^3: generating inputs")
+
+    def case(seed):
+        d, s_, w = synth.snapshot_inputs_torch(N, seed, dev)
+        hist = torch.stack([w * float(((1 + z) / (1 + zz[max(snap - k, 0)])) ** 3) for k in range(nh)])
+        return d, s_, w, hist
+
+    fc = np.zeros(len(zz))
+    fc[:snap] = np.linspace(1e-4, 5e-3, snap)
+    tsp = P.ts_params(N, L, snap, zz, fc, nh, efficiency=20.0, on_device=1)
+    hp = P.hii_params(N, L, z, filter_type=0, efficiency=20.0, uvb_flag=0, include_spin_temp=1, on_device=1)
+    bp = P.tb_params(N, z, include_spin_temp=1, on_device=1)
+    xe0 = None
+    Tk0 = None
+    box = L / P.HUBBLE_H
+    kf = MA.k_factor_for_bins(N, 20)
+    d2, s2, w2, h2 = case(1)
+    xe0 = torch.full_like(d2, 2e-3)
+    Tk0 = torch.full((N, N, N), 50.0, device=dev)
+    dT2, _ = MA.snapshot(ctx, tsp, hp, bp, d2, s2, w2, w2, h2, xe0, Tk0)
+    Pref = MA.power_spectrum(ctx, dT2.detach(), box, kf).detach()
+    del d2, s2, w2, h2, dT2
+    d, s_, w, hist = case(0)
+    t_f, t_b = [], []
+    for it in range(steps + 1):
+        dx = d.clone().requires_grad_(True)
+        sx = s_.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dT, _ = MA.snapshot(ctx, tsp, hp, bp, dx, sx, w, w, hist, xe0, Tk0)
+        Pk = MA.power_spectrum(ctx, dT, box, kf)
+        loss = ((Pk - Pref) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it > 0:  # the first pass warms up
+            t_f.append(t1 - t0)
+            t_b.append(t2 - t1)
+        g_norm = float(dx.grad.double().norm())
+        del dx, sx, dT, Pk, loss
+    tf, tb = float(np.median(t_f)), float(np.median(t_b))
+    log(f"[bench] adjoint {N}^3: forward {tf * 1e3:.1f} ms, backward {tb * 1e3:.1f} ms, ratio {tb / tf:.2f} "
+        f"(|dL/d deltax| = {g_norm:.3e})")
+    return {"forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "backward_over_forward": tb / tf,
+            "loss": "sum over 20 log-k bins of (P_dTb - P_ref)^2, P_ref from a second seed",
+            "grads": ["deltax", "stars"]}
 
 
 BATCH_SIZES = (64, 128, 192)
@@ -204,8 +299,20 @@ def main():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["snapshot", "bubbles", "batched"], default="snapshot")
-    ap.add_argument("--no-extras", action="store_true", help="skip the batched line of the snapshot run")
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched parts of the snapshot run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 3 times (scale of the PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the reionization path runs the whole box on a single MI355X "
               f"(--gpus {args.gpus} is not supported; multi-GPU sharding is design-only)", file=sys.stderr)
@@ -309,8 +416,15 @@ def main():
     avg_s = ms_tot / cnt / 1e3
     b = kernel_bytes(name, N, F)
     roof = {"kernel": name, "bound": "hbm", "achieved": (b / avg_s / 1e9) if b else None, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": (b / avg_s / HBM_PEAK) if b else None, "traffic": None,
+            "unit": "GB/s", "frac": (b / avg_s / HBM_PEAK) if b else None, "traffic": measured_traffic(name, N),
             "avg_launch_us": avg_s * 1e6, "bytes_per_launch": b}
+    per_kernel = {}
+    for k, (ms, n) in prof.items():
+        kb = kernel_bytes(k, N, F)
+        per_kernel[k] = {"ms_per_step": ms / args.steps, "launches_per_step": n // args.steps,
+                         "us_per_launch": ms / n * 1e3,
+                         "GB_s": (kb * n / (ms / 1e3) / 1e9) if kb else None,
+                         "frac": (kb * n / (ms / 1e3) / HBM_PEAK) if kb else None}
     Bc = 8 * N * N * (N // 2 + 1)
     mb = model_bytes_hii(N, F, S, True) + 16 * N**3
     if snapshot:
@@ -337,9 +451,12 @@ def main():
         "model_hbm_frac": mb / dt / HBM_PEAK,
         "model_bytes": mb,
         "roofline": roof,
-        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+        "kernels": per_kernel,
     }
     if snapshot and not args.no_extras:
+        del dd, ds, dw, hist, xH, rb, J21, dT, zin, J21i, xe0, Tk0, xe_prev, Tk, xe_box, TS
+        torch.cuda.empty_cache()
+        out["adjoint"] = {str(n): bench_adjoint(ctx, n) for n in (256, N)}
         out["batched"] = bench_batched(ctx, 2, 1)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")

@@ -66,12 +66,14 @@ __device__ __forceinline__ XeBracket xe_bracket(const TsDev& t, float xe0)
   return b;
 }
 
-__device__ __forceinline__ double interp(const double (*tab)[kTsMaxSteps], int ct, const XeBracket& b, double* deriv)
+// f = slope * fr + f[lo], as the forward kernel forms it (tab/slope at the
+// cell's bracket and shell); deriv = d f / d x_e
+__device__ __forceinline__ double interp(double tab, double slope, const XeBracket& b, double* deriv)
 {
-  double f = (tab[b.hi][ct] - tab[b.lo][ct]) / b.dxh;
+  double f = slope;
   *deriv = f * b.dfr;
   f *= b.fr;
-  f += tab[b.lo][ct];
+  f += tab;
   return f;
 }
 
@@ -89,19 +91,21 @@ using D7 = Dual<7>;
 // One cell of the inhomogeneous spin-temperature step in dual numbers
 // (directions: 0 delta, 1 x_e_prev, 2 T_K_prev, 3..6 the raw shell sums
 // heat, ion, lya, star-lya); returns T_S and writes the new x_e.
-__device__ __forceinline__ D7 ts_cell_dual(const TsDev& t, float dl, float xe0, float Tk0, const float* shells,
-                                           long ncell, long i, D7* xn_out)
+__device__ __forceinline__ D7 ts_cell_dual(const TsDev& t, const double* tabs, float dl, float xe0, float Tk0,
+                                           const float* shells, long ncell, long i, D7* xn_out)
 {
   const XeBracket br = xe_bracket(t, xe0);
+  const int ns = t.nsteps, per = kNXHII * ns;
   double H = 0, I = 0, Ly = 0, St = 0, dH = 0, dI = 0, dLy = 0;
   if (!t.no_light) {
     for (int ct = 0; ct < t.nsteps; ++ct) {
       const float sv = shells[(long)ct * ncell + i];
       const double SFR = ((double)sv / t.pixel_volume) * t.MT * t.L3 / 1.989e33;
       double dfh, dfi, dfl;
-      const double fh = interp(t.fheat, ct, br, &dfh);
-      const double fi = interp(t.fion, ct, br, &dfi);
-      const double fl = interp(t.flya, ct, br, &dfl);
+      const int at = br.lo * ns + ct;
+      const double fh = interp(tabs[at], tabs[3 * per + at], br, &dfh);
+      const double fi = interp(tabs[per + at], tabs[4 * per + at], br, &dfi);
+      const double fl = interp(tabs[2 * per + at], tabs[5 * per + at], br, &dfl);
       const double dt_dzpp = t.dt_dzpp[ct], dzpp = t.dzpp[ct];
       const double zpp_integrand = SFR * t.zpp_pow[ct];
       H += dt_dzpp * dzpp * zpp_integrand * fh;
@@ -162,13 +166,9 @@ __global__ __launch_bounds__(kNT) void k_snapshot_adjoint_cell(
   const float* __restrict__ gdT, float* __restrict__ g_delta, float* __restrict__ g_stars,
   float* __restrict__ g_xe_prev, float* __restrict__ g_Tk, double* __restrict__ gS, AdjCellArgs a)
 {
-  __shared__ TsDev t;
-  if (a.ts_mode != 0) {
-    const int words = (int)(sizeof(TsDev) / 4);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(tg);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&t);
-    for (int k = threadIdx.x; k < words; k += kNT) dst[k] = src[k];
-  }
+  extern __shared__ double tabs[];  // the x_e tables when ts_mode == 2 (ts_cell_lds_bytes)
+  const TsDev& t = *tg;
+  if (a.ts_mode == 2) load_ts_tables(t, tabs);
   __syncthreads();
   const int N = a.N;
   const long ncell = (long)N * N * N;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kNT) void k_snapshot_adjoint_cell(
   double gxe0 = 0.0, gT0 = 0.0;
   if (a.ts_mode == 2) {
     D7 xn;
-    const D7 tsd = ts_cell_dual(t, dl, xe_prev[ip], Tk_prev[i], shells, ncell, i, &xn);
+    const D7 tsd = ts_cell_dual(t, tabs, dl, xe_prev[ip], Tk_prev[i], shells, ncell, i, &xn);
     double g[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) g[k] = gTS * tsd.d[k] + gxe_new * xn.d[k];
@@ -238,12 +238,14 @@ __global__ __launch_bounds__(kNT) void k_ts_shell_grad(const TsDev* __restrict__
                                                       const double* __restrict__ gS, float* __restrict__ out, int ct,
                                                       int N)
 {
-  __shared__ TsDev t;
-  {
-    const int words = (int)(sizeof(TsDev) / 4);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(tg);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&t);
-    for (int k = threadIdx.x; k < words; k += kNT) dst[k] = src[k];
+  // this shell's column of the six x_e tables
+  __shared__ double col[6][kNXHII];
+  const TsDev& t = *tg;
+  if (threadIdx.x < 6 * kNXHII) {
+    const int tb = threadIdx.x / kNXHII, x = threadIdx.x % kNXHII;
+    const double(*src)[kTsMaxSteps] =
+      tb == 0 ? t.fheat : tb == 1 ? t.fion : tb == 2 ? t.flya : tb == 3 ? t.sheat : tb == 4 ? t.sion : t.slya;
+    col[tb][x] = src[x][ct];
   }
   __syncthreads();
   const long ncell = (long)N * N * N;
@@ -257,9 +259,9 @@ __global__ __launch_bounds__(kNT) void k_ts_shell_grad(const TsDev* __restrict__
   if (sv > 0.0f) {
     const XeBracket br = xe_bracket(t, xe_prev[ip]);
     double dd;
-    const double fh = interp(t.fheat, ct, br, &dd);
-    const double fi = interp(t.fion, ct, br, &dd);
-    const double fl = interp(t.flya, ct, br, &dd);
+    const double fh = interp(col[0][br.lo], col[3][br.lo], br, &dd);
+    const double fi = interp(col[1][br.lo], col[4][br.lo], br, &dd);
+    const double fl = interp(col[2][br.lo], col[5][br.lo], br, &dd);
     const double dSFR = ((1.0 / t.pixel_volume) * t.MT * t.L3) / 1.989e33;
     const double w = t.dt_dzpp[ct] * t.dzpp[ct] * t.zpp_pow[ct];
     const double wl = t.P2 * t.zpp1[ct] * t.sum_lyn[ct] * t.dt_dzpp[ct] * t.dzpp[ct];

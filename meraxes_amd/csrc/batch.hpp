@@ -142,7 +142,7 @@ template <int N>
 struct BatchPipe {
   static constexpr int NH = N / 2;
   static constexpr long nc = (long)N * N * NH;
-  static int nb_threshold() { return N * N / Cfg<N>::RZ; }
+  static int nb_threshold() { return N * N / ThrCfg<N>::RZ; }
 
   struct Fwd {
     mrx_ctx* ctx;
@@ -168,14 +168,15 @@ struct BatchPipe {
     int go()
     {
       using M = TileMap<N, TW>;
+      using MX = TileMap<N, TW, kNT>;
       const dim3 gx(N * M::NKZB, c->nbox);
       const long bs = (long)c->F * nc, bvs = (long)c->F * N * N;
       if (win)
-        k_ix_filter<N, true, TW, SV><<<gx, M::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc, (long)N * N,
-                                                                 c->F, bs, bvs);
+        k_ix_filter<N, true, TW, SV><<<gx, MX::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc, (long)N * N,
+                                                                  c->F, bs, bvs);
       else
-        k_ix_filter<N, false, TW, SV><<<gx, M::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc,
-                                                                  (long)N * N, c->F, bs, bvs);
+        k_ix_filter<N, false, TW, SV><<<gx, MX::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc,
+                                                                   (long)N * N, c->F, bs, bvs);
       MRX_TRY(check_launch(ctx, "k_ix_filter (batch)"));
       const dim3 gy(N * M::NKZB, c->nbox * c->F);
       k_colpass<N, true, TW, SV><<<gy, M::NT, 0, c->stream>>>(c->T, c->tw, nc, (long)N * NH, NH, 0.0);
@@ -227,8 +228,13 @@ struct BatchPipe {
       a.R_cubed = c.R[s] * c.R[s] * c.R[s];
       a.step = s;
       a.last = last ? 1 : 0;
-      k_c2r_threshold_batch<N><<<dim3(nb, c.nbox), kNT, 0, c.stream>>>(c.T, nc, (long)c.F * nc, c.tw, c.boxes_dev,
-                                                                       c.state, c.Rtab, c.J21c, c.S, c.partials, a);
+      constexpr int NT = ThrCfg<N>::NT;
+      if (c.F == 4)
+        k_c2r_threshold_batch<N, 4><<<dim3(nb, c.nbox), NT, 0, c.stream>>>(
+          c.T, nc, (long)c.F * nc, c.tw, c.boxes_dev, c.state, c.Rtab, c.J21c, c.S, c.partials, a);
+      else
+        k_c2r_threshold_batch<N, 3><<<dim3(nb, c.nbox), NT, 0, c.stream>>>(
+          c.T, nc, (long)c.F * nc, c.tw, c.boxes_dev, c.state, c.Rtab, c.J21c, c.S, c.partials, a);
       MRX_TRY(check_launch(ctx, "k_c2r_threshold_batch"));
     }
     k_reduce_partials<<<c.nbox, kNT, 0, c.stream>>>(c.partials, nb, 4, c.sums);

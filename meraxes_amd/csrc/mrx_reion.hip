@@ -61,6 +61,7 @@ struct mrx_ctx {
     hipEvent_t a, b;
   };
   bool prof = false;
+  bool prof_ts = false;  // launches belong to the spin-temperature shell loop (names get a "ts_" prefix)
   std::vector<ProfRec> prof_recs;
   std::vector<hipEvent_t> event_pool;
   // spin temperature: host tables (reloaded when the key changes) and device buffers
@@ -243,7 +244,7 @@ struct ProfScope {
     }
   }
 };
-#define PROF(name) ProfScope prof_scope_##__LINE__(ctx, name)
+#define PROF(name) ProfScope prof_scope_##__LINE__(ctx, ctx->prof_ts ? "ts_" name : name)
 
 int ensure_buf(mrx_ctx* ctx, float*& p, size_t& cap, size_t nfloat)
 {
@@ -345,9 +346,9 @@ struct Pipe {
     {
     PROF("ix_filter");
     if (Wt)
-      k_ix_filter<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
+      k_ix_filter<N, true><<<gc, Cfg<N>::IX_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
     else
-      k_ix_filter<N, false><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
+      k_ix_filter<N, false><<<gc, Cfg<N>::IX_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter"));
     const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
@@ -367,14 +368,19 @@ struct Pipe {
   static int threshold(mrx_ctx* ctx, const float* deltax, float* xH, float* rb, float* J21, float* zin, float* J21i,
                        const StepArgs& a)
   {
-    const int nb = N * N / Cfg<N>::RZ;
+    const int nb = N * N / ThrCfg<N>::RZ;
+    constexpr int NT = ThrCfg<N>::NT;
     PROF("c2r_threshold");
-    k_c2r_threshold<N><<<nb, kNT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, zin, J21i,
-                                                    ctx->Rtab, ctx->partials, a);
+    if (a.nfield == 4)
+      k_c2r_threshold<N, 4><<<nb, NT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, zin,
+                                                        J21i, ctx->Rtab, ctx->partials, a);
+    else
+      k_c2r_threshold<N, 3><<<nb, NT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, zin,
+                                                        J21i, ctx->Rtab, ctx->partials, a);
     return check_launch(ctx, "k_c2r_threshold");
   }
 
-  static int nblocks_threshold() { return N * N / Cfg<N>::RZ; }
+  static int nblocks_threshold() { return N * N / ThrCfg<N>::RZ; }
 
   static int pack_herm(mrx_ctx* ctx, const float2* H, float2* U)
   {
@@ -612,6 +618,12 @@ void fill_ts_dev(TsDev& t, const mrx_ts_data* d, const mrx_ts_snap* s, const mrx
       t.fion[x][c] = s->freq_ion[x][c];
       t.flya[x][c] = s->freq_lya[x][c];
     }
+    for (int x = 0; x + 1 < kNXHII; ++x) {
+      const double dxh = (double)(d->xi_XHII[x + 1] - d->xi_XHII[x]);  // float difference, as the reference
+      t.sheat[x][c] = (s->freq_heat[x + 1][c] - s->freq_heat[x][c]) / dxh;
+      t.sion[x][c] = (s->freq_ion[x + 1][c] - s->freq_ion[x][c]) / dxh;
+      t.slya[x][c] = (s->freq_lya[x + 1][c] - s->freq_lya[x][c]) / dxh;
+    }
   }
   for (int x = 0; x < kNXHII; ++x) t.XH[x] = d->xi_XHII[x];
   const float zp = (float)s->zp;
@@ -670,6 +682,11 @@ struct TsRun {
                    bool prepare_only = false)
     {
       using P = Pipe<N>;
+      struct TsTag {  // kernel timings of this step are named "ts_..."
+        mrx_ctx* c;
+        explicit TsTag(mrx_ctx* x) : c(x) { c->prof_ts = true; }
+        ~TsTag() { c->prof_ts = false; }
+      } tag(ctx);
       mrx_ts_data* d = ctx->tsd;
       mrx_ts_snap* s = ctx->tss;
       const long ncell = (long)N * N * N;
@@ -678,6 +695,7 @@ struct TsRun {
       const double box_size = p->box_size / h;
       const double pixel_volume = pow(box_size / (double)N, 3);
       const int nbc = (int)((ncell + kNT - 1) / kNT);
+      int nred = nbc;  // workgroups whose 7 partial sums are reduced at the end
       MRX_TRY(ensure_partials(ctx, (size_t)nbc * 7 + 2 * 1024 + 16));
       if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
       if (!ctx->ts_dev) HIP_TRY(hipMalloc(&ctx->ts_dev, sizeof(TsDev)));
@@ -691,7 +709,7 @@ struct TsRun {
         fa.N = N;
         fa.with_xe = with_xe;
         {
-          PROF("ts_fcoll");
+          PROF("fcoll");
           k_ts_fcoll<<<1024, kNT, 0, ctx->stream>>>(stars, deltax, xe_prev, ctx->partials + (size_t)nbc * 7, fa);
         }
         MRX_TRY(check_launch(ctx, "k_ts_fcoll"));
@@ -714,7 +732,7 @@ struct TsRun {
         ctx->ts_h_T0 = T0;
         if (prepare_only) return MRX_OK;
         {
-          PROF("ts_homogeneous");
+          PROF("homogeneous");
           k_ts_homogeneous<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, deltax, xe_prev, Tk, TS, N, xe0, T0,
                                                          ctx->partials);
         }
@@ -732,7 +750,7 @@ struct TsRun {
             const mrx_sfr_op& o = s->ops[ct][k];
             const long n = (o.kind <= 1) ? half : ncell;
             const float* hs = (o.kind >= 2) ? hist + (size_t)o.slot * np : nullptr;
-            PROF("ts_sfr_op");
+            PROF("sfr_op");
             k_sfr_op<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(ctx->ts_work, hs, o.kind, o.weight,
                                                                           (float)o.weight, N, half);
             MRX_TRY(check_launch(ctx, "k_sfr_op"));
@@ -757,13 +775,13 @@ struct TsRun {
         HIP_TRY(hipMemcpyAsync(ctx->ts_dev, ctx->ts_host, sizeof(TsDev), hipMemcpyHostToDevice, ctx->stream));
         if (prepare_only) return MRX_OK;
         {
-          PROF("ts_cell");
-          k_ts_cell<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, ctx->ts_shells, deltax, xe_prev, Tk, TS, N,
-                                                  ctx->partials);
+          PROF("cell");
+          k_ts_cell<<<nred, kNT, ts_cell_lds_bytes(s->nsteps), ctx->stream>>>(ctx->ts_dev, ctx->ts_shells, deltax,
+                                                                               xe_prev, Tk, TS, N, ctx->partials);
         }
         MRX_TRY(check_launch(ctx, "k_ts_cell"));
       }
-      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nbc, 7, ctx->sums);
+      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nred, 7, ctx->sums);
       MRX_TRY(check_launch(ctx, "k_reduce_partials"));
       double hs[7];
       HIP_TRY(hipMemcpyAsync(hs, ctx->sums, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
@@ -969,7 +987,7 @@ struct SnapAdj {
       const unsigned nbc = (unsigned)((ncell + kNT - 1) / kNT);
       {
         PROF("adjoint_cell");
-        k_snapshot_adjoint_cell<<<nbc, kNT, 0, ctx->stream>>>(
+        k_snapshot_adjoint_cell<<<nbc, kNT, a.ts_mode == 2 ? ts_cell_lds_bytes(s->nsteps) : 0, ctx->stream>>>(
           ctx->ts_dev, ctx->ts_shells, deltax, stars, xe_prev, Tk, xH, TS, xe_box, gdT, g_delta, g_stars,
           a.ts_mode == 2 ? g_xe_prev : nullptr, a.ts_mode == 2 ? g_Tk : nullptr, ctx->adj_gS, a);
       }

@@ -29,6 +29,7 @@ namespace mrx {
 constexpr int kTW = 16;        // pencils per strided tile
 constexpr int kZE = 2048;      // complex elements per z-pass workgroup
 constexpr int kNT = 256;       // threads per workgroup
+constexpr int kColNT = 512;    // at most this many threads per strided tile
 constexpr float kRelTol = 1e-5f;  // REL_TOL (src/meraxes.h:41)
 constexpr float kAbsTol = 1e-8f;  // ABS_TOL (src/meraxes.h:42)
 
@@ -40,7 +41,8 @@ struct Cfg {
   static constexpr int NH = N / 2;
   static constexpr int TW = (NH % kTW == 0) ? kTW : kTW / 2;  // pencils per strided tile
   static constexpr int COL_NT_RAW = TW * N / 8;
-  static constexpr int COL_NT = COL_NT_RAW < 64 ? 64 : (COL_NT_RAW > kNT ? kNT : COL_NT_RAW);
+  static constexpr int COL_NT = COL_NT_RAW < 64 ? 64 : (COL_NT_RAW > kColNT ? kColNT : COL_NT_RAW);
+  static constexpr int IX_NT = COL_NT_RAW < 64 ? 64 : (COL_NT_RAW > kNT ? kNT : COL_NT_RAW);
   static constexpr int RZ = (NH >= kZE) ? 1 : pow2_floor(kZE / NH);  // rows per z workgroup (divides N*N)
   static constexpr int ZE = RZ * NH;                               // complex elements per z workgroup
   static constexpr int CPT = 2 * ZE / kNT;                          // real cells per thread in the z pass
@@ -182,15 +184,15 @@ __global__ __launch_bounds__(kNT) void k_r2c_z_tab(const float* const* __restric
 }
 
 // ---------------------------------------------------- z: C2R load + FFT
-// Loads RZ packed rows, untwists them and runs the inverse length-N/2 FFT in
-// LDS; afterwards the LDS tile, read as floats, holds RZ real rows of N.
-template <int N>
-__device__ __forceinline__ void c2r_untwist_fft(float2* s, const float2* tw, int tid)
+// Untwists ROWS packed rows held in LDS and runs the inverse length-N/2 FFT
+// on all of them with NT threads; afterwards the tile, read as floats, holds
+// ROWS real rows of N.
+template <int N, int ROWS, int NT>
+__device__ __forceinline__ void c2r_untwist_rows(float2* s, const float2* tw, int tid)
 {
-  using C = Cfg<N>;
-  constexpr int NH = C::NH, RZ = C::RZ;
+  constexpr int NH = N / 2;
   constexpr int NP = NH / 2 + 1;
-  for (int q = tid; q < RZ * NP; q += kNT) {
+  for (int q = tid; q < ROWS * NP; q += NT) {
     const int r = q / NP, k = q % NP;
     const int rb = r * NH;
     if (k == 0) {
@@ -205,7 +207,13 @@ __device__ __forceinline__ void c2r_untwist_fft(float2* s, const float2* tw, int
     }
   }
   __syncthreads();
-  fft_lds<NH, RZ, kNT, true, 1, NH, false, 2, true>(s, tw, tid);
+  fft_lds<NH, ROWS, NT, true, 1, NH, false, 2, true>(s, tw, tid);
+}
+
+template <int N>
+__device__ __forceinline__ void c2r_untwist_fft(float2* s, const float2* tw, int tid)
+{
+  c2r_untwist_rows<N, Cfg<N>::RZ, kNT>(s, tw, tid);
 }
 
 // Loads RZ packed rows, untwists them and runs the inverse length-N/2 FFT in
@@ -277,11 +285,11 @@ __global__ __launch_bounds__(kNT) void k_c2r_z(const float2* __restrict__ in, fl
 // Thread map of an N x TW tile: thread t owns column t % TW and rows
 // t / TW + i * RPI, i < EPT.  Every load of a tile is issued before the first
 // is consumed, so a wave keeps EPT row segments in flight.
-template <int N, int TW_>
+template <int N, int TW_, int MAXNT = kColNT>
 struct TileMap {
   static constexpr int TW = TW_;
   static constexpr int NT_RAW = TW * N / 8;
-  static constexpr int NT = NT_RAW < 64 ? 64 : (NT_RAW > kNT ? kNT : NT_RAW);
+  static constexpr int NT = NT_RAW < 64 ? 64 : (NT_RAW > MAXNT ? MAXNT : NT_RAW);
   static constexpr int SP = TW + 2;  // LDS row pitch, breaks the bank pattern
   static constexpr int RPI = NT / TW;
   static constexpr int EPT = N / RPI;
@@ -294,7 +302,7 @@ struct TileMap {
 // `rstride`; tile t covers kz block (t mod NKZB) of line (t / NKZB), line
 // stride astride; blockIdx.y selects the field (stride fstride).
 template <int N, bool INV, int TW = Cfg<N>::TW, int SV = 0>
-__global__ __launch_bounds__((TileMap<N, TW>::NT)) void k_colpass(float2* __restrict__ data,
+__global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass(float2* __restrict__ data,
                                                                const float2* __restrict__ twN, long fstride,
                                                                long astride, long rstride, double div)
 {
@@ -353,7 +361,7 @@ __global__ __launch_bounds__(kNT) void k_window_table(float* __restrict__ Wt, Wi
 // tile is loaded while the current one is transformed.  blockIdx.y selects a
 // box of a batch (strides bstride for U/T, bvstride for V).
 template <int N, bool WIN, int TW = Cfg<N>::TW, int SV = 0>
-__global__ __launch_bounds__((TileMap<N, TW>::NT)) void k_ix_filter(const float2* __restrict__ U,
+__global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const float2* __restrict__ U,
                                                                  const float2* __restrict__ V,
                                                                  float2* __restrict__ T,
                                                                  const float2* __restrict__ twN,
@@ -361,7 +369,7 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT)) void k_ix_filter(const float2
                                                                  long vstride, int nfield, long bstride,
                                                                  long bvstride)
 {
-  using M = TileMap<N, TW>;
+  using M = TileMap<N, TW, kNT>;  // 256 threads: the window values stay in registers
   constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
   constexpr int NH = N / 2, NH1 = NH + 1;
   constexpr unsigned rs = (unsigned)N * NH;
@@ -548,7 +556,25 @@ struct StepArgs {
 // byte (0xFF = never crossed, else the step of the latest crossing); xH,
 // r_bubble and the J_21 zeros are written once, on the last step, and the
 // first crossing writes J_21 / z_at_ionization / J_21_at_ionization directly.
+// Threads and 4-cell groups of the fused z pass + threshold: all NF fields of
+// RZ rows sit in LDS together and are transformed as NF*RZ pencils, so the
+// fields share every barrier of the transform.
+// Power-of-two grids take 1024 complex elements of rows per field (32 KiB of
+// LDS for four fields, so four workgroups share a CU); the others keep the
+// z pass's row blocks.
 template <int N>
+struct ThrCfg {
+  static constexpr int NH = N / 2;
+  static constexpr bool P2 = (Cfg<N>::ZE % 1024 == 0) && NH <= 1024;
+  static constexpr int RZ = P2 ? 1024 / NH : Cfg<N>::RZ;
+  static constexpr int ZE = RZ * NH;
+  static constexpr int NT = 256;
+  static constexpr int G = ZE / (2 * NT);  // groups of 4 cells per thread
+  static_assert(ZE % (2 * NT) == 0, "z tile must split into 4-cell groups");
+  static_assert((N * N) % RZ == 0, "threshold workgroups must tile the rows");
+};
+
+template <int N, int NF>
 __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T, long fstride,
                                                    const float2* __restrict__ twN,
                                                    const float* __restrict__ deltax_in,
@@ -559,56 +585,56 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
                                                    const StepArgs& a)
 {
 #pragma clang fp contract(off)
-  using C = Cfg<N>;
-  constexpr int NH = C::NH, RZ = C::RZ, ZE = C::ZE, CPT = C::CPT, EPZ = ZE / kNT;
-  __shared__ float2 s[ZE + ZE / 32];
+  using TC = ThrCfg<N>;
+  constexpr int NH = TC::NH, RZ = TC::RZ, ZE = TC::ZE, NT = TC::NT, G = TC::G;
+  constexpr int TOT = NF * ZE;  // complex elements of every field's rows
+  constexpr int EPT = TOT / NT;
+  static_assert(TOT % NT == 0, "fields must split evenly over the threads");
+  __shared__ float2 s[TOT + TOT / 32];
   __shared__ float2 tw[N];
-  __shared__ double red[kNT];
+  __shared__ double red[NT];
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += kNT) tw[i] = twN[i];
   const long row0 = (long)blockIdx.x * RZ;
-  // issue every field's loads up front so they overlap the transforms
-  float2 pre[4][EPZ];
+  {
+    float2 v[EPT];
 #pragma unroll
-  for (int f = 0; f < 4; ++f)
-    if (f < a.nfield) {
-      const float2* src = T + f * fstride + row0 * NH;
-#pragma unroll
-      for (int i = 0; i < EPZ; ++i) pre[f][i] = src[tid + i * kNT];
+    for (int i = 0; i < EPT; ++i) {
+      const int e = tid + i * NT, f = e / ZE, k = e % ZE;
+      v[i] = T[f * fstride + row0 * NH + k];
     }
-  // thread t owns the cell groups 4 * (t + j * kNT) + q, q < 4, j < CPT / 4:
-  // four consecutive cells of one row, so state is one word and the outputs
-  // are 16-byte stores
-  constexpr int G = CPT / 4;
+    for (int i = tid; i < N; i += NT) tw[i] = twN[i];
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) s[lds_addr<true>(tid + i * NT)] = v[i];
+  }
+  __syncthreads();
+  c2r_untwist_rows<N, NF * RZ, NT>(s, tw, tid);
+  // thread t owns the cell groups 4 * (t + j * NT) + q, q < 4, j < G: four
+  // consecutive cells of one row, so state is one word and the outputs are
+  // 16-byte stores
+  constexpr int CPT = 4 * G;
   float val[4][CPT];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
-    if (f < a.nfield) {
+    const int kind = (f == 0) ? 0 : (f == 3 ? 2 : 1);
 #pragma unroll
-      for (int i = 0; i < EPZ; ++i) s[lds_addr<true>(tid + i * kNT)] = pre[f][i];
-      __syncthreads();
-      c2r_untwist_fft<N>(s, tw, tid);
-      const int kind = (f == 0) ? 0 : (f == 3 ? 2 : 1);
-#pragma unroll
-      for (int j = 0; j < G; ++j) {
-        const int h = 2 * (tid + j * kNT);  // float2 index of the group's first pair
+    for (int j = 0; j < G; ++j) {
+      if (f < NF) {
+        const int h = f * ZE + 2 * (tid + j * NT);  // float2 index of the group's first pair
         const float2 p0 = s[lds_addr<true>(h)], p1 = s[lds_addr<true>(h + 1)];
         val[f][4 * j + 0] = clamp_kind(p0.x, kind);
         val[f][4 * j + 1] = clamp_kind(p0.y, kind);
         val[f][4 * j + 2] = clamp_kind(p1.x, kind);
         val[f][4 * j + 3] = clamp_kind(p1.y, kind);
+      } else {
+        val[f][4 * j + 0] = val[f][4 * j + 1] = val[f][4 * j + 2] = val[f][4 * j + 3] = 0.0f;
       }
-      __syncthreads();
-    } else {
-#pragma unroll
-      for (int i = 0; i < CPT; ++i) val[f][i] = 0.0f;
     }
   }
   double s_xH = 0.0, s_xHm = 0.0, s_m = 0.0, s_J = 0.0;
   const long cell0 = row0 * N;
 #pragma unroll
   for (int j = 0; j < G; ++j) {
-    const int e0 = 4 * (tid + j * kNT);
+    const int e0 = 4 * (tid + j * NT);
     const long ir0 = cell0 + e0;
     const uint32_t st_old = (a.step == 0) ? 0xFFFFFFFFu : *reinterpret_cast<const uint32_t*>(state + ir0);
     uint32_t st_new = st_old;
@@ -693,10 +719,10 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   }
   if (a.last) {
     double* p = partials + (long)blockIdx.x * 4;
-    double v0 = block_sum<kNT>(s_xH, red);
-    double v1 = block_sum<kNT>(s_xHm, red);
-    double v2 = block_sum<kNT>(s_m, red);
-    double v3 = block_sum<kNT>(s_J, red);
+    double v0 = block_sum<NT>(s_xH, red);
+    double v1 = block_sum<NT>(s_xHm, red);
+    double v2 = block_sum<NT>(s_m, red);
+    double v3 = block_sum<NT>(s_J, red);
     if (tid == 0) {
       p[0] = v0;
       p[1] = v1;
@@ -706,8 +732,8 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   }
 }
 
-template <int N>
-__global__ __launch_bounds__(kNT) void k_c2r_threshold(const float2* __restrict__ T, long fstride,
+template <int N, int NF>
+__global__ __launch_bounds__(ThrCfg<N>::NT) void k_c2r_threshold(const float2* __restrict__ T, long fstride,
                                                       const float2* __restrict__ twN,
                                                       const float* __restrict__ deltax_in,
                                                       uint8_t* __restrict__ state, float* __restrict__ xH,
@@ -716,7 +742,8 @@ __global__ __launch_bounds__(kNT) void k_c2r_threshold(const float2* __restrict_
                                                       const float* __restrict__ Rtab, double* __restrict__ partials,
                                                       StepArgs a)
 {
-  c2r_threshold_tile<N>(T, fstride, twN, deltax_in, state, xH, r_bubble, J21, z_in, J21_ion, Rtab, partials, a);
+  c2r_threshold_tile<N, NF>(T, fstride, twN, deltax_in, state, xH, r_bubble, J21, z_in, J21_ion, Rtab, partials,
+                            a);
 }
 
 // Per-box grids and scalars of a batch (device array, one entry per box).
@@ -735,8 +762,8 @@ struct BoxDev {
 // The ladder step of every box of a cohort: blockIdx.y is the box.  T holds
 // nfield spectra per box (box stride bstride); state and partials are
 // contiguous per box; J21c_tab[box * nstep + step] is the box's J_21 constant.
-template <int N>
-__global__ __launch_bounds__(kNT) void k_c2r_threshold_batch(const float2* __restrict__ T, long fstride, long bstride,
+template <int N, int NF>
+__global__ __launch_bounds__(ThrCfg<N>::NT) void k_c2r_threshold_batch(const float2* __restrict__ T, long fstride, long bstride,
                                                             const float2* __restrict__ twN,
                                                             const BoxDev* __restrict__ boxes,
                                                             uint8_t* __restrict__ state,
@@ -750,7 +777,7 @@ __global__ __launch_bounds__(kNT) void k_c2r_threshold_batch(const float2* __res
   a.redshift = bx.redshift;
   a.bias = bx.bias;
   a.J21c = J21c_tab[(long)box * nstep + a.step];
-  c2r_threshold_tile<N>(T + box * bstride, fstride, twN, bx.deltax, state + (long)box * N * N * N, bx.xH,
+  c2r_threshold_tile<N, NF>(T + box * bstride, fstride, twN, bx.deltax, state + (long)box * N * N * N, bx.xH,
                         bx.r_bubble, bx.J21, bx.z_in, bx.J21_ion, Rtab,
                         partials + (long)box * gridDim.x * 4, a);
 }

@@ -38,6 +38,10 @@ struct TsDev {
   // per-shell factors
   double dt_dzpp[kTsMaxSteps], dzpp[kTsMaxSteps], zpp1[kTsMaxSteps], zpp_pow[kTsMaxSteps], sum_lyn[kTsMaxSteps];
   double fheat[kNXHII][kTsMaxSteps], fion[kNXHII][kTsMaxSteps], flya[kNXHII][kTsMaxSteps];
+  // slopes of the x_e interpolation, (f[lo+1] - f[lo]) / (XH[lo+1] - XH[lo]):
+  // the quotient the reference forms per cell, formed once on the host with
+  // the same double operation, so it is bit-identical
+  double sheat[kNXHII][kTsMaxSteps], sion[kNXHII][kTsMaxSteps], slya[kNXHII][kTsMaxSteps];
   float XH[kNXHII];
   // snapshot constants
   double dzp, dt_dzp, const_zp_prefactor, growth, dgrowth;
@@ -182,23 +186,42 @@ __device__ __forceinline__ double dev_alpha_A(double T) { return alpha_A_t<doubl
 
 // Per-cell spin-temperature update (ComputeTs.c:743-900); 7 block partial sums:
 // J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e.
+// Dynamic LDS: the six x_e-interpolation tables (values, slopes of heat, ion,
+// Ly-alpha), [6][kNXHII][nsteps] doubles, the only constants indexed per lane;
+// everything else is read uniformly from the constants in global memory.
+constexpr size_t ts_cell_lds_bytes(int nsteps) { return sizeof(double) * 6 * kNXHII * (size_t)nsteps; }
+
+__device__ __forceinline__ void load_ts_tables(const TsDev& t, double* tabs)
+{
+  const int ns = t.nsteps, per = kNXHII * ns;
+  for (int k = threadIdx.x; k < 6 * per; k += blockDim.x) {
+    const int tb = k / per, rem = k % per, x = rem / ns, c = rem % ns;
+    const double(*src)[kTsMaxSteps] =
+      tb == 0 ? t.fheat : tb == 1 ? t.fion : tb == 2 ? t.flya : tb == 3 ? t.sheat : tb == 4 ? t.sion : t.slya;
+    tabs[k] = src[x][c];
+  }
+}
+
 __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, const float* __restrict__ shells,
                                                 const float* __restrict__ deltax, float* __restrict__ xe_prev,
                                                 float* __restrict__ Tk, float* __restrict__ TS, int N,
                                                 double* __restrict__ partials)
 {
-  __shared__ TsDev t;
+  extern __shared__ double tabs[];
   __shared__ double red[kNT];
-  {
-    const int words = (int)(sizeof(TsDev) / 4);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(tg);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&t);
-    for (int i = threadIdx.x; i < words; i += kNT) dst[i] = src[i];
-  }
+  const TsDev& t = *tg;
+  const int ns = t.nsteps;
+  load_ts_tables(t, tabs);
   __syncthreads();
+  const double* fheat = tabs;
+  const double* fion = tabs + kNXHII * ns;
+  const double* flya = tabs + 2 * kNXHII * ns;
+  const double* sheat = tabs + 3 * kNXHII * ns;
+  const double* sion = tabs + 4 * kNXHII * ns;
+  const double* slya = tabs + 5 * kNXHII * ns;
   const long ncell = (long)N * N * N;
-  const long i = (long)blockIdx.x * kNT + threadIdx.x;
   double sJ = 0, sxa = 0, sXh = 0, sXi = 0, sTS = 0, sTK = 0, sxe = 0;
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
   if (i < ncell) {
     const long r = i / N;
     const int z = (int)(i % N);
@@ -215,8 +238,6 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
     int lo = kNXHII - 1;
     const float xc_f = (float)xHII_call;
     while (xc_f < t.XH[lo]) lo--;
-    const int hi = lo + 1;
-    const double dxh = (double)(t.XH[hi] - t.XH[lo]);
     const double fr = xHII_call - t.XH[lo];
     const double n_b = t.NBP3 * (1 + dl);
     double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
@@ -224,15 +245,16 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
       for (int ct = 0; ct < t.nsteps; ++ct) {
         const float sv = shells[(long)ct * ncell + i];
         const double SFR = ((double)sv / t.pixel_volume) * t.MT * t.L3 / 1.989e33;
-        double fh = (t.fheat[hi][ct] - t.fheat[lo][ct]) / dxh;
+        const int at = lo * ns + ct;
+        double fh = sheat[at];
         fh *= fr;
-        fh += t.fheat[lo][ct];
-        double fi = (t.fion[hi][ct] - t.fion[lo][ct]) / dxh;
+        fh += fheat[at];
+        double fi = sion[at];
         fi *= fr;
-        fi += t.fion[lo][ct];
-        double fl = (t.flya[hi][ct] - t.flya[lo][ct]) / dxh;
+        fi += fion[at];
+        double fl = slya[at];
         fl *= fr;
-        fl += t.flya[lo][ct];
+        fl += flya[at];
         const double dt_dzpp = t.dt_dzpp[ct], dzpp = t.dzpp[ct];
         const double zpp_integrand = SFR * t.zpp_pow[ct];
         dxheat += dt_dzpp * dzpp * zpp_integrand * fh;

@@ -197,7 +197,9 @@ def _ts_loss(ctx, c, d, s, Pref, box, kf, grad=False):
 
 
 def test_snapshot_adjoint_with_ts_finite_differences(ctx):
-    N, L = 64, 50.0
+    # 32^3: a one-cell perturbation moves the float32 forward's loss by
+    # ~eps / N^1.5 per mode against ~1e-7 of rounding, so FD is meaningful here
+    N, L = 32, 25.0
     box = L / P.HUBBLE_H
     c = _ts_case(N, L, 21)
     # measure the tables' grid means once, then hold them fixed
@@ -223,7 +225,10 @@ def test_snapshot_adjoint_with_ts_finite_differences(ctx):
     stars_n = c["s"].cpu().numpy()[:, :, :N]
     ok = (margin >= 1e-3) & (xH > 0) & (xH < 1)
     cand_d = np.argwhere(ok)
-    cand_s = np.argwhere(ok & (stars_n > 0))  # stars enter through max(v, 0): perturb positive cells
+    # xH is affine in a cell's own stars, so the central difference is exact up
+    # to rounding for any step that flips nothing: h = 1e-3 (internal mass
+    # units, ~10% of a cell's stars) on cells 1e-2 away from the threshold
+    cand_s = np.argwhere((margin >= 1e-2) & (xH > 0) & (xH < 1) & (stars_n > 2e-3))
     pick_d = cand_d[rng.choice(len(cand_d), size=64, replace=False)]
     pick_s = cand_s[rng.choice(len(cand_s), size=64, replace=False)]
     eps = 1e-3
@@ -235,7 +240,7 @@ def test_snapshot_adjoint_with_ts_finite_differences(ctx):
                                                     ("s", fd_s, ad_s, gs_n, pick_s[j])):
             x, y, zc = (int(v) for v in cell)
             base = c[which]
-            h = eps if which == "d" else eps * float(base[x, y, zc])
+            h = eps
             vals = []
             for sgn in (1.0, -1.0):
                 pert = base.clone()
