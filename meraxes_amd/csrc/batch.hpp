@@ -163,12 +163,18 @@ struct BatchPipe {
   struct InvXY {
     mrx_ctx* ctx;
     Cohort* c;
-    bool win;
+    const WinArgs* w;  // NULL: the unfiltered last step
     template <int TW, int SV>
     int go()
     {
       using M = TileMap<N, TW>;
       using MX = TileMap<N, TW, kNT>;
+      const bool win = (w != nullptr);
+      if (win) {  // the table layout follows the tile width
+        const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
+        k_window_table<N, TW><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, c->stream>>>(c->wtab, *w);
+        MRX_TRY(check_launch(ctx, "k_window_table (batch)"));
+      }
       const dim3 gx(N * M::NKZB, c->nbox);
       const long bs = (long)c->F * nc, bvs = (long)c->F * N * N;
       if (win)
@@ -195,16 +201,9 @@ struct BatchPipe {
     return check_launch(ctx, "k_unpack_dc (batch)");
   }
 
-  static int window(mrx_ctx* ctx, Cohort& c, const WinArgs& w)
+  static int inverse_xy(mrx_ctx* ctx, Cohort& c, const WinArgs* w, int variant)
   {
-    const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
-    k_window_table<N><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, c.stream>>>(c.wtab, w);
-    return check_launch(ctx, "k_window_table (batch)");
-  }
-
-  static int inverse_xy(mrx_ctx* ctx, Cohort& c, bool win, int variant)
-  {
-    InvXY f{ctx, &c, win};
+    InvXY f{ctx, &c, w};
     return with_variant<N>(variant, f);
   }
 
@@ -222,8 +221,8 @@ struct BatchPipe {
     const int nb = nb_threshold();
     for (int s = 0; s < c.S; ++s) {
       const bool last = (s == c.S - 1);
-      if (!last) MRX_TRY(window(ctx, c, win_args(p->box_size, (float)c.R[s], p->filter_type, 1)));
-      MRX_TRY(inverse_xy(ctx, c, !last, c.variant));
+      const WinArgs w = win_args(p->box_size, (float)c.R[s], p->filter_type, 1);
+      MRX_TRY(inverse_xy(ctx, c, last ? nullptr : &w, c.variant));
       a.M_mean = RtoM(p, c.R[s]);
       a.R_cubed = c.R[s] * c.R[s] * c.R[s];
       a.step = s;
@@ -299,7 +298,7 @@ struct BatchPipe {
     return MRX_OK;
   }
 
-  // Times every variant's filtered inverse x+y passes over the whole cohort
+  // Times every variant's window table + filtered inverse x+y passes over the whole cohort
   // (the radius of step 0, real spectra from one forward transform) and keeps
   // the fastest.
   static int tune(mrx_ctx* ctx, Cohort& c)
@@ -310,7 +309,7 @@ struct BatchPipe {
     if (ncand <= 1) return MRX_OK;
     c.variant = 0;
     MRX_TRY(forward(ctx, c));
-    MRX_TRY(window(ctx, c, win_args(c.p0.box_size, (float)c.R[0], c.p0.filter_type, 1)));
+    const WinArgs w0 = win_args(c.p0.box_size, (float)c.R[0], c.p0.filter_type, 1);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -318,9 +317,9 @@ struct BatchPipe {
     int bestv = 0;
     for (int v = 0; v < kNumVariants; ++v) {
       if (!variant_exists<N>(v)) continue;
-      MRX_TRY(inverse_xy(ctx, c, true, v));  // warm-up
+      MRX_TRY(inverse_xy(ctx, c, &w0, v));  // warm-up
       HIP_TRY(hipEventRecord(e0, c.stream));
-      for (int r = 0; r < 3; ++r) MRX_TRY(inverse_xy(ctx, c, true, v));
+      for (int r = 0; r < 3; ++r) MRX_TRY(inverse_xy(ctx, c, &w0, v));
       HIP_TRY(hipEventRecord(e1, c.stream));
       HIP_TRY(hipEventSynchronize(e1));
       float ms = 0.f;

@@ -337,7 +337,7 @@ struct Pipe {
       MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt));
       {
         PROF("window_table");
-        k_window_table<N><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->wtab, w);
+        k_window_table<N, Cfg<N>::TW><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->wtab, w);
       }
       MRX_TRY(check_launch(ctx, "k_window_table"));
       Wt = ctx->wtab;

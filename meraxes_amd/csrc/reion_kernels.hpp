@@ -339,18 +339,24 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass(float2* __r
 
 // Window table of one radius: W depends on (|nx|, |ny|, nz) only, because
 // filter() builds each k component as n*dk or (n-N)*dk and squares it, so
-// modes n and N-n give the same float k^2.  Entry (a, b, c) holds
-// W(a, b, c) for a, b, c in [0, N/2].
-template <int N>
+// modes n and N-n give the same float k^2.  Layout for tile width TW (kz
+// blocks of TW): W(a, b, c) for c < N/2 at ((b * NKZB + c / TW) * NH1 + a) *
+// TW + c % TW, so the x-pass tile (|ky| = b, kz block) reads one contiguous
+// NH1 x TW block; the Nyquist column c = N/2 follows at NH1*(N/2)*NH1 + b*NH1 + a.
+template <int N, int TW>
 __global__ __launch_bounds__(kNT) void k_window_table(float* __restrict__ Wt, WinArgs w)
 {
-  constexpr int NH1 = N / 2 + 1;
+  constexpr int NH = N / 2, NH1 = NH + 1, NKZB = NH / TW;
   const long idx = (long)blockIdx.x * kNT + threadIdx.x;
   if (idx >= (long)NH1 * NH1 * NH1) return;
   const int c = (int)(idx % NH1);
   const long ab = idx / NH1;
   const int b = (int)(ab % NH1), a = (int)(ab / NH1);
-  Wt[idx] = window(a, b, c, N, w);
+  const float v = window(a, b, c, N, w);
+  if (c < NH)
+    Wt[(((long)b * NKZB + c / TW) * NH1 + a) * TW + c % TW] = v;
+  else
+    Wt[(long)NH1 * NH * NH1 + (long)b * NH1 + a] = v;
 }
 
 // Inverse x pass with the window applied on load, for every field of the
@@ -389,15 +395,15 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
 #pragma unroll
     for (int i = 0; i < EPT; ++i) r[i] = u[(unsigned)(i * RPI) * rs];
   }
-  // W(|kx|, |ky|, kz) = Wt[(|kx| * NH1 + |ky|) * NH1 + kz]
+  // this tile's block of the window table (k_window_table's layout)
   float w[WIN ? EPT : 1];
   if constexpr (WIN) {
-    const float* wr = Wt + (long)aky * NH1 + b * TW + c;
+    const float* wr = Wt + ((long)aky * NKZB + b) * NH1 * TW + c;
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
       const int n = n0 + i * RPI;
       const int an = n <= NH ? n : N - n;
-      w[i] = wr[(unsigned)an * (NH1 * NH1)];
+      w[i] = wr[(unsigned)an * TW];
     }
   }
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
@@ -410,7 +416,7 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
       const float2* vp = V + f * vstride + (long)ky * N;
       for (int n = tid; n < N; n += NT) {
         const int an = n <= NH ? n : N - n;
-        const float wn = WIN ? Wt[((unsigned)an * NH1 + aky) * NH1 + NH] : 1.0f;
+        const float wn = WIN ? Wt[(long)NH1 * NH * NH1 + (long)aky * NH1 + an] : 1.0f;
         const float2 nb = cscale(vp[n], wn);
         const float2 v = s[n * SP];
         s[n * SP] = make_float2(v.x - nb.y, v.y + nb.x);
@@ -559,14 +565,15 @@ struct StepArgs {
 // Threads and 4-cell groups of the fused z pass + threshold: all NF fields of
 // RZ rows sit in LDS together and are transformed as NF*RZ pencils, so the
 // fields share every barrier of the transform.
-// Power-of-two grids take 1024 complex elements of rows per field (32 KiB of
-// LDS for four fields, so four workgroups share a CU); the others keep the
-// z pass's row blocks.
+// Power-of-two grids take 512 complex elements of rows per field (16 KiB of
+// LDS for four fields: measured at 512^3, 512 beats 1024 by 10% and 2048 by
+// 38%, more resident workgroups hiding the transform's barriers); the others
+// keep the z pass's row blocks.
 template <int N>
 struct ThrCfg {
   static constexpr int NH = N / 2;
-  static constexpr bool P2 = (Cfg<N>::ZE % 1024 == 0) && NH <= 1024;
-  static constexpr int RZ = P2 ? 1024 / NH : Cfg<N>::RZ;
+  static constexpr bool P2 = (Cfg<N>::ZE % 1024 == 0) && NH <= 512;
+  static constexpr int RZ = P2 ? 512 / NH : Cfg<N>::RZ;
   static constexpr int ZE = RZ * NH;
   static constexpr int NT = 256;
   static constexpr int G = ZE / (2 * NT);  // groups of 4 cells per thread
