@@ -271,6 +271,27 @@ def bench_batched(ctx, steps, warmup, per_size=BATCH_PER_SIZE):
     t_plain = time.perf_counter() - t0
     log(f"[bench] per-box plain loop: {t_plain * 1e3:.2f} ms")
     batch.close()
+    # the same comparison one grid size at a time
+    per_size = {}
+    for N in BATCH_SIZES:
+        sub = [b for b in boxes if b["params"].grid_dim == N]
+        bs = reion.HiiBatch(ctx, sub, graph=True, tune=True)
+        bs.run(sync=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bs.run(sync=False)
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for b in bs.boxes:
+            reion.find_hii_bubbles(ctx, b["params"], b["deltax"], b["stars"], b["weighted_sfr"],
+                                   z_at_ionization=b["z_at_ionization"], out=b)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        bs.close()
+        per_size[str(N)] = {"batched_ms": tb * 1e3, "plain_loop_ms": tp * 1e3, "speedup": tp / tb}
+        log(f"[bench] batched {N}^3 x {len(sub)}: {tb * 1e3:.2f} ms, plain loop {tp * 1e3:.2f} ms, x{tp / tb:.2f}")
     return {
         "metric": "cells*radii/s of find_HII_bubbles over 96 uneven boxes, batched (one graph launch)",
         "value": cells_radii / t_batch,
@@ -283,6 +304,7 @@ def bench_batched(ctx, steps, warmup, per_size=BATCH_PER_SIZE):
         "model_bytes": mbytes,
         "cells_radii": cells_radii,
         "plan_s": t_plan,
+        "per_grid": per_size,
         "graph_nodes": info["graph_nodes"],
         "cohorts": [{k: c[k] for k in ("grid_dim", "boxes", "radii", "variant", "tile_width", "radix_schedule")}
                     for c in info["cohorts"]],

@@ -232,11 +232,14 @@ __global__ __launch_bounds__(kNT) void k_snapshot_adjoint_cell(
 
 // Gradient of shell ct, dL/d(shell_ct) on the padded grid: the raw sums'
 // adjoints times their coefficients at the cell's x_e, where the forward's
-// max(v, 0) clamp passed the value.
+// max(v, 0) clamp passed the value (v > 0; the derivative is taken as 0 at
+// v = 0).  `unclamped`, when given, is the padded grid v itself (the
+// unfiltered shell 0 is the working grid, whose zeros the transform's
+// round-off would otherwise make positive or negative at random).
 __global__ __launch_bounds__(kNT) void k_ts_shell_grad(const TsDev* __restrict__ tg, const float* __restrict__ shells,
                                                       const float* __restrict__ xe_prev,
                                                       const double* __restrict__ gS, float* __restrict__ out, int ct,
-                                                      int N)
+                                                      int N, const float* __restrict__ unclamped)
 {
   // this shell's column of the six x_e tables
   __shared__ double col[6][kNXHII];
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(kNT) void k_ts_shell_grad(const TsDev* __restrict__
   const long ip = r * (N + 2) + z;
   const float sv = shells[(long)ct * ncell + i];
   double g = 0.0;
-  if (sv > 0.0f) {
+  if ((unclamped ? unclamped[ip] : sv) > 0.0f) {
     const XeBracket br = xe_bracket(t, xe_prev[ip]);
     double dd;
     const double fh = interp(col[0][br.lo], col[3][br.lo], br, &dd);

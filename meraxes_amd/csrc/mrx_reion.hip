@@ -1005,10 +1005,23 @@ struct SnapAdj {
       float* tmp = ctx->adj_work + np;
       const long half = (long)N * N * (N / 2 + 1);
       for (int ct = s->nsteps - 1; ct >= 0; --ct) {
+        const float* unclamped = nullptr;
+        if (ct == 0) {  // shell 0 is the working grid after its operations: rebuild it
+          HIP_TRY(hipMemcpyAsync(tmp, sfr, sizeof(float) * np, hipMemcpyDeviceToDevice, ctx->stream));
+          for (int k = 0; k < s->n_ops[0]; ++k) {
+            const mrx_sfr_op& o = s->ops[0][k];
+            const long n = (o.kind <= 1) ? half : ncell;
+            const float* hs = (o.kind >= 2) ? hist + (size_t)o.slot * np : nullptr;
+            k_sfr_op<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(tmp, hs, o.kind, o.weight,
+                                                                          (float)o.weight, N, half);
+            MRX_TRY(check_launch(ctx, "k_sfr_op"));
+          }
+          unclamped = tmp;
+        }
         {
           PROF("adjoint_shell_grad");
           k_ts_shell_grad<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, ctx->ts_shells, xe_prev, ctx->adj_gS, gshell, ct,
-                                                        N);
+                                                        N, unclamped);
         }
         MRX_TRY(check_launch(ctx, "k_ts_shell_grad"));
         const float* add = gshell;

@@ -68,6 +68,7 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
 /* _ComputeTs restated (ts.c).  Same pointer contract as mrx_compute_ts (host
  * memory); shells_out, if not NULL, receives the num_filter_steps smoothed SFR
  * shells (float, shell-major, unpadded) of the inhomogeneous branch. */
+int orc_ts_export(const mrx_ts_params* p, double* out, int cap);
 int orc_compute_ts(const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
                    const float* sfr_histories, float* x_e_box, float* x_e_box_prev, float* Tk_box, float* TS_box,
                    mrx_ts_globals* out, float* shells_out);

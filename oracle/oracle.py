@@ -46,6 +46,7 @@ def lib():
         L.orc_radius_ladder.argtypes = [C.POINTER(HiiParams), vp, i]
         L.orc_brightness_temp.argtypes = [C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
         L.orc_compute_ts.argtypes = [C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals), vp]
+        L.orc_ts_export.argtypes = [C.POINTER(TsParams), vp, i]
         L.orc_set_threads.argtypes = [i]
         L.orc_get_threads.restype = i
         _lib = L
@@ -190,4 +191,40 @@ def compute_ts(p, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box, want_
                               C.byref(g), fptr(shells)), "compute_ts")
     out = {k: getattr(g, k) for k, _ in TsGlobals._fields_}
     out.update({"x_e_box": xe, "x_e_box_prev": xe_prev, "Tk_box": Tk, "TS_box": TS, "shells": shells})
+    return out
+
+
+def ts_export(p):
+    """Constants of the inhomogeneous spin-temperature step (p.fixed_globals must be set)."""
+    buf = np.zeros(200000, np.float64)
+    n = lib().orc_ts_export(C.byref(p), buf.ctypes.data_as(C.c_void_p), len(buf))
+    _chk(0 if n > 0 else n, "ts_export")
+    o = buf[:n]
+    ns = int(o[0])
+    out = {"nsteps": ns, "no_light": int(o[1]), "zp": o[2], "dzp": o[3], "dt_dzp": o[4],
+           "const_zp_prefactor": o[5], "growth": o[6], "dgrowth": o[7], "N_b0": o[8], "No": o[9], "f_H": o[10],
+           "f_He": o[11], "H": o[12], "hubble_h": o[13]}
+    nop = int(o[14])
+    k = 16
+    for name in ("R", "dt_dzpp", "dzpp", "zpp", "zpp_pow", "sum_lyn"):
+        out[name] = o[k:k + ns].copy()
+        k += ns
+    for name in ("freq_heat", "freq_ion", "freq_lya"):
+        out[name] = o[k:k + 14 * ns].reshape(14, ns).copy()
+        k += 14 * ns
+    out["XH"] = o[k:k + 14].astype(np.float32)
+    k += 14
+    for name in ("kHH", "keH", "kpH"):
+        m = int(o[k])
+        k += 1
+        sp = {}
+        for f in ("x", "y", "b", "c", "d"):
+            sp[f] = o[k:k + m].copy()
+            k += 27
+        out[name] = sp
+    ops = []
+    for _ in range(nop):
+        ops.append((int(o[k]), int(o[k + 1]), int(o[k + 2]), float(o[k + 3])))
+        k += 4
+    out["ops"] = ops
     return out

@@ -451,3 +451,88 @@ int orc_ts_plan_summary(const mrx_ts_params* p, double* R, double* zpp_edge, int
   free(d);
   return rc;
 }
+
+/* Snapshot constants of the inhomogeneous spin-temperature step, flattened
+ * for test references (float64 restatements of the per-cell update).  Needs
+ * p->fixed_globals (the tables' two grid means come from p).  Layout:
+ *   [0] nsteps [1] no_light [2] zp [3] dzp [4] dt_dzp [5] const_zp_prefactor
+ *   [6] growth_factor_zp [7] dgrowth_factor_dzp [8] N_b0 [9] No [10] f_H
+ *   [11] f_He [12] hubble((float)zp) [13] hubble_h [14] n_op
+ *   [16 ...] R, dt_dzpp, dzpp, zpp, zpp_pow, sum_lyn (nsteps each),
+ *   freq_heat, freq_ion, freq_lya ([14][nsteps] each), XH[14],
+ *   3 kappa splines (HH, eH, pH): n, x[n], y[n], b[n], c[n], d[n],
+ *   n_op ops: shell, kind, slot, weight.
+ * Returns the number of doubles written, or a negative code. */
+int orc_ts_export(const mrx_ts_params* p, double* out, int cap)
+{
+  if (!p->fixed_globals) return MRX_EINVAL;
+  mrx_ts_data* d = (mrx_ts_data*)orc_ts_data_new(p);
+  if (!d) return MRX_EINVAL;
+  mrx_ts_snap* s = (mrx_ts_snap*)calloc(1, sizeof(mrx_ts_snap));
+  int rc = mrx_ts_plan(s, d, p);
+  if (rc == MRX_OK && s->homogeneous) rc = MRX_EUNSUPPORTED;
+  if (rc == MRX_OK) {
+    s->collapse_fraction = p->fixed_collapse_fraction;
+    s->x_e_ave = p->fixed_x_e_ave;
+    rc = mrx_ts_build_tables(s, d, p);
+  }
+  int n = 0;
+  const int ns = s->nsteps;
+  int nop = 0;
+  for (int c = 0; c < ns; c++) nop += s->n_ops[c];
+  const int need = 16 + 6 * ns + 3 * MRX_X_INT_NXHII * ns + MRX_X_INT_NXHII + 3 * (1 + 5 * 27) + 4 * nop;
+  if (rc == MRX_OK && cap < need) rc = MRX_EINVAL;
+  if (rc == MRX_OK) {
+    double* o = out;
+    o[0] = ns;
+    o[1] = s->no_light;
+    o[2] = s->zp;
+    o[3] = s->dzp;
+    o[4] = s->dt_dzp;
+    o[5] = s->const_zp_prefactor;
+    o[6] = s->growth_factor_zp;
+    o[7] = s->dgrowth_factor_dzp;
+    o[8] = d->N_b0;
+    o[9] = d->No;
+    o[10] = d->f_H;
+    o[11] = d->f_He;
+    o[12] = mrx_ts_hubble(d, (float)s->zp);
+    o[13] = p->hubble_h;
+    o[14] = nop;
+    o[15] = 0;
+    n = 16;
+    for (int c = 0; c < ns; c++) out[n++] = s->R[c];
+    for (int c = 0; c < ns; c++) out[n++] = s->dt_dzpp[c];
+    for (int c = 0; c < ns; c++) out[n++] = s->dzpp[c];
+    for (int c = 0; c < ns; c++) out[n++] = s->zpp[c];
+    for (int c = 0; c < ns; c++) out[n++] = s->zpp_pow[c];
+    for (int c = 0; c < ns; c++) out[n++] = s->sum_lyn[c];
+    for (int x = 0; x < MRX_X_INT_NXHII; x++)
+      for (int c = 0; c < ns; c++) out[n++] = s->freq_heat[x][c];
+    for (int x = 0; x < MRX_X_INT_NXHII; x++)
+      for (int c = 0; c < ns; c++) out[n++] = s->freq_ion[x][c];
+    for (int x = 0; x < MRX_X_INT_NXHII; x++)
+      for (int c = 0; c < ns; c++) out[n++] = s->freq_lya[x][c];
+    for (int x = 0; x < MRX_X_INT_NXHII; x++) out[n++] = d->xi_XHII[x];
+    const mrx_spline* sp[3] = {&d->kappa_HH, &d->kappa_eH, &d->kappa_pH};
+    for (int k = 0; k < 3; k++) {
+      const int m = sp[k]->n < 27 ? sp[k]->n : 27;
+      out[n++] = m;
+      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->x[i] : 0.0;
+      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->y[i] : 0.0;
+      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->b[i] : 0.0;
+      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->c[i] : 0.0;
+      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->d[i] : 0.0;
+    }
+    for (int c = 0; c < ns; c++)
+      for (int k = 0; k < s->n_ops[c]; k++) {
+        out[n++] = c;
+        out[n++] = s->ops[c][k].kind;
+        out[n++] = s->ops[c][k].slot;
+        out[n++] = s->ops[c][k].weight;
+      }
+  }
+  free(s);
+  free(d);
+  return rc == MRX_OK ? n : rc;
+}

@@ -8,7 +8,9 @@
   through a power-spectrum loss, every cell;
 * with spin temperature: central finite differences (eps = 1e-3) of the GPU
   forward's loss on 64 random cells whose threshold margin is at least 1e-3,
-  with the shell tables' grid means fixed in both runs.
+  with the shell tables' grid means fixed in both runs; and every gradient
+  (deltax, stars, sfr, SFR history, x_e_prev, T_K) against float64 autograd of
+  an independent torch restatement (tests/ts_torch.py) at 64^3.
 """
 import numpy as np
 import pytest
@@ -258,3 +260,76 @@ def test_snapshot_adjoint_with_ts_finite_differences(ctx):
           f"{np.median(np.abs(ad_d - fd_d) / np.maximum(np.abs(fd_d), 1e-30)):.2e}), stars rel L2 {e_s:.2e}",
           flush=True)
     assert e_d < 5e-2 and e_s < 5e-2
+
+
+def test_snapshot_adjoint_with_ts_matches_float64_autograd(ctx):
+    """Every gradient of the spin-temperature snapshot against torch float64 autograd at 64^3.
+
+    The reference (tests/ts_torch.py) rebuilds the shells from sfr and the SFR
+    history, the per-cell Ts step, the partially ionised xH (the GPU forward's
+    mask held fixed), delta_T and the power-spectrum loss in float64 from the
+    oracle's snapshot constants, sharing no code with the HIP adjoint.
+    """
+    import ts_torch as TT
+
+    N, L = 64, 50.0
+    box = L / P.HUBBLE_H
+    c = _ts_case(N, L, 31)
+    g0 = reion.compute_ts(ctx, c["tsp"], c["d"], c["s"], c["w"], c["hist"], c["xe0"].clone(), c["Tk0"].clone())
+    tsp = c["tsp"]
+    tsp.fixed_globals = 1
+    tsp.fixed_collapse_fraction = g0["collapse_fraction"]
+    tsp.fixed_x_e_ave = g0["x_e_ave"]
+    cst = O.ts_export(tsp)
+    assert cst["nsteps"] == 40 and not cst["no_light"] and len(cst["ops"]) > 0
+    c2 = _ts_case(N, L, 32)
+    kf = MA.k_factor_for_bins(N, 12)
+    dT2, _ = MA.snapshot(ctx, tsp, c2["hp"], c2["bp"], c2["d"], c2["s"], c2["w"], c2["w"], c2["hist"], c2["xe0"],
+                         c2["Tk0"])
+    Pref = MA.power_spectrum(ctx, dT2.detach(), box, kf).detach()
+    # GPU: hand-written adjoint
+    leaves = {k: v.clone().requires_grad_(True) for k, v in
+              (("d", c["d"]), ("s", c["s"]), ("sfr", c["w"]), ("hist", c["hist"]), ("xe0", c["xe0"]),
+               ("Tk0", c["Tk0"]))}
+    dT, aux = MA.snapshot(ctx, tsp, c["hp"], c["bp"], leaves["d"], leaves["s"], c["w"], leaves["sfr"],
+                          leaves["hist"], leaves["xe0"], leaves["Tk0"])
+    loss = ((MA.power_spectrum(ctx, dT, box, kf) - Pref) ** 2).sum()
+    loss.backward()
+    xH_g, TS_g = aux[0].double(), aux[1].double()
+    # float64 reference
+    dev = c["d"].device
+    r = {k: v.detach().double().clone().requires_grad_(True) for k, v in
+         (("d", c["d"]), ("s", c["s"]), ("sfr", c["w"]), ("hist", c["hist"]), ("xe0", c["xe0"]), ("Tk0", c["Tk0"]))}
+    h = tsp.hubble_h
+    pixel_volume = (tsp.box_size / h / N) ** 3
+    MT = tsp.unit_mass_g / tsp.unit_time_s
+    L3 = tsp.unit_length_cm ** -3.0
+    sh = TT.shells(cst, r["sfr"], r["hist"], N, tsp.box_size, dev)
+    d64 = r["d"][:, :, :N]
+    xn, Tn, TS = TT.ts_step(cst, d64, r["xe0"][:, :, :N], r["Tk0"], sh, pixel_volume, MT, L3)
+    eTS = rel_l2(TS.detach().cpu().numpy(), TS_g.cpu().numpy())
+    hp = c["hp"]
+    R = P.radius_ladder(hp)[-1]
+    fc_unit = (4.0 / 3.0) * np.pi * R**3 / (L / N) ** 3 / ((4.0 / 3.0) * np.pi * R**3 * hp.omega_m * hp.rho_crit)
+    s64 = r["s"][:, :, :N]
+    s64 = torch.where(s64 < 1e-8, torch.zeros_like(s64), s64)  # the stars clamp (ABS_TOL)
+    xe_c = torch.where(xn < 1e-8, torch.zeros_like(xn), torch.clamp(xn, max=0.999))
+    xpart = torch.clamp(1.0 - xe_c - hp.efficiency * s64 * fc_unit / (1.0 + d64), 0.0, 1.0)
+    act = (xH_g > 0) & (xH_g < 1)
+    xH64 = torch.where(act, xpart, xH_g)
+    zf = float(np.float32(hp.redshift))
+    dT64 = tb_const(hp.redshift) * xH64 * (1.0 + d64) * (1.0 - 2.728 * (1.0 + zf) / TS)
+    bins, kmag, nb = ps_bins(N, box, kf)
+    L64 = ((ps_torch(dT64, box, kf, bins, kmag, nb) - Pref.double()) ** 2).sum()
+    L64.backward()
+    print(f"partial cells {act.double().mean():.3f}; T_S float64 vs GPU rel L2 {eTS:.2e}; "
+          f"loss gpu {float(loss.detach()):.6e} float64 {float(L64.detach()):.6e}", flush=True)
+    assert eTS < 1e-5
+    errs = {}
+    for k in ("d", "s", "sfr", "hist", "xe0", "Tk0"):
+        g = leaves[k].grad.detach().double().cpu().numpy()
+        ref = r[k].grad.detach().cpu().numpy()
+        errs[k] = rel_l2(g, ref)
+    print("gradient rel L2 vs float64 autograd: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v < 1e-4, k
