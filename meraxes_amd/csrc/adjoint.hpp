@@ -100,7 +100,7 @@ __device__ __forceinline__ D7 ts_cell_dual(const TsDev& t, const double* tabs, f
   if (!t.no_light) {
     for (int ct = 0; ct < t.nsteps; ++ct) {
       const float sv = shells[(long)ct * ncell + i];
-      const double SFR = ((double)sv / t.pixel_volume) * t.MT * t.L3 / 1.989e33;
+      const double SFR = (double)sv * t.sfr_factor;
       double dfh, dfi, dfl;
       const int at = br.lo * ns + ct;
       const double fh = interp(tabs[at], tabs[3 * per + at], br, &dfh);
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kNT) void k_ts_shell_grad(const TsDev* __restrict__
     const double fh = interp(col[0][br.lo], col[3][br.lo], br, &dd);
     const double fi = interp(col[1][br.lo], col[4][br.lo], br, &dd);
     const double fl = interp(col[2][br.lo], col[5][br.lo], br, &dd);
-    const double dSFR = ((1.0 / t.pixel_volume) * t.MT * t.L3) / 1.989e33;
+    const double dSFR = t.sfr_factor;
     const double w = t.dt_dzpp[ct] * t.dzpp[ct] * t.zpp_pow[ct];
     const double wl = t.P2 * t.zpp1[ct] * t.sum_lyn[ct] * t.dt_dzpp[ct] * t.dzpp[ct];
     g = dSFR * (gS[i] * w * fh + gS[ncell + i] * w * fi + gS[2 * ncell + i] * w * fl + gS[3 * ncell + i] * wl);

@@ -607,6 +607,7 @@ void fill_ts_dev(TsDev& t, const mrx_ts_data* d, const mrx_ts_snap* s, const mrx
   t.pixel_volume = pow(box_size / (double)N, 3);
   t.MT = (p->unit_mass_g / p->unit_time_s);
   t.L3 = pow(p->unit_length_cm, -3.);
+  t.sfr_factor = (1.0 / t.pixel_volume) * t.MT * t.L3 / 1.989e33;
   for (int c = 0; c < s->nsteps; ++c) {
     t.dt_dzpp[c] = s->dt_dzpp[c];
     t.dzpp[c] = s->dzpp[c];

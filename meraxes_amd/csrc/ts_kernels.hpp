@@ -5,8 +5,9 @@
 // space, and the clamped shell (float) is stored shell-major, [shell][cell],
 // so the per-cell kernel reads one coalesced row per shell.  The reference
 // keeps the shells as double (ComputeTs.c:420-433); the double it stores is a
-// fixed product of the float shell value, so the per-cell kernel rebuilds the
-// identical double from the float and the store costs 4 instead of 8 bytes.
+// fixed product of the float shell value, so the per-cell kernel rebuilds that
+// double from the float (to within 2 ulp, one folded factor instead of the
+// reference's two divisions) and the store costs 4 instead of 8 bytes.
 //
 // The per-cell kernel is ComputeTs.c:743-900 with evolveInt
 // (XRayHeatingFunctions.c:1241-1435) and get_Ts (:1513-1558) in FP64; every
@@ -33,8 +34,10 @@ struct KappaSpline {
 struct TsDev {
   int nsteps;
   int no_light;
-  // shell value -> SMOOTHED_SFR double: ((f / V) * MT) * L3 / MSUN
-  double pixel_volume, MT, L3;
+  // shell value -> SMOOTHED_SFR double, the reference's ((f / V) * MT) * L3 / MSUN
+  // (ComputeTs.c:420-433) folded into one factor, (1/V) MT L3 / MSUN: a
+  // multiply instead of two FP64 divisions per shell and cell, within 2 ulp
+  double pixel_volume, MT, L3, sfr_factor;
   // per-shell factors
   double dt_dzpp[kTsMaxSteps], dzpp[kTsMaxSteps], zpp1[kTsMaxSteps], zpp_pow[kTsMaxSteps], sum_lyn[kTsMaxSteps];
   double fheat[kNXHII][kTsMaxSteps], fion[kNXHII][kTsMaxSteps], flya[kNXHII][kTsMaxSteps];
@@ -242,9 +245,10 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
     const double n_b = t.NBP3 * (1 + dl);
     double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
     if (!t.no_light) {
+#pragma unroll 4
       for (int ct = 0; ct < t.nsteps; ++ct) {
         const float sv = shells[(long)ct * ncell + i];
-        const double SFR = ((double)sv / t.pixel_volume) * t.MT * t.L3 / 1.989e33;
+        const double SFR = (double)sv * t.sfr_factor;
         const int at = lo * ns + ct;
         double fh = sheat[at];
         fh *= fr;
