@@ -129,6 +129,24 @@ def cpu_baseline(snapshot):
     }
 
 
+def copy_ceiling(torch, nbytes=1 << 30, reps=10):
+    """Practical HBM ceiling: a 1 GiB device-to-device copy, bytes read + written per second."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    del src, dst
+    return {"GB_s": 2 * nbytes / t / 1e9, "frac_of_peak": 2 * nbytes / t / HBM_PEAK, "bytes": nbytes,
+            "kernel": "torch copy_ (1 GiB, 10 repetitions, CUDA events on the current stream)"}
+
+
 def measured_traffic(kernel, N):
     """HBM bytes per launch of `kernel` from the committed PMC summary, or None.
 
@@ -475,6 +493,8 @@ def main():
         "roofline": roof,
         "kernels": per_kernel,
     }
+    out["hbm_copy_ceiling"] = copy_ceiling(torch)
+    log(f"[bench] device copy ceiling {out['hbm_copy_ceiling']['GB_s']:.0f} GB/s (read + write)")
     if snapshot and not args.no_extras:
         del dd, ds, dw, hist, xH, rb, J21, dT, zin, J21i, xe0, Tk0, xe_prev, Tk, xe_box, TS
         torch.cuda.empty_cache()

@@ -36,7 +36,7 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     # host-side spin-temperature tables: plain C, no fused multiply-adds
     obj = os.path.join(CSRC, "ts_tables.o")
-    ccmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall", "-c",
+    ccmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-c",
             os.path.join(CSRC, "ts_tables.c"), "-o", obj]
     if verbose:
         print(" ".join(ccmd), flush=True)
@@ -45,7 +45,7 @@ def build(verbose=False):
         raise RuntimeError(f"gcc failed:\n{r.stderr}")
     # the HIP translation unit and the C object are linked together through
     # the linker so hipcc does not treat the object as a HIP source
-    cmd = [hipcc()] + flags() + ["-o", tmp, src, "-Xlinker", obj]
+    cmd = [hipcc()] + flags() + ["-o", tmp, src, "-Xlinker", obj, "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -178,11 +178,12 @@ struct BatchPipe {
       const dim3 gx(N * M::NKZB, c->nbox);
       const long bs = (long)c->F * nc, bvs = (long)c->F * N * N;
       if (win)
-        k_ix_filter<N, true, TW, SV><<<gx, MX::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc, (long)N * N,
-                                                                  c->F, bs, bvs);
+        k_ix_filter<N, kWinTable, TW, SV><<<gx, MX::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc,
+                                                                       (long)N * N, c->F, bs, bvs, *w, N, M::NKZB);
       else
-        k_ix_filter<N, false, TW, SV><<<gx, MX::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc,
-                                                                   (long)N * N, c->F, bs, bvs);
+        k_ix_filter<N, kWinNone, TW, SV><<<gx, MX::NT, 0, c->stream>>>(c->U, c->V, c->T, c->tw, c->wtab, nc,
+                                                                      (long)N * N, c->F, bs, bvs, WinArgs{}, N,
+                                                                      M::NKZB);
       MRX_TRY(check_launch(ctx, "k_ix_filter (batch)"));
       const dim3 gy(N * M::NKZB, c->nbox * c->F);
       k_colpass<N, true, TW, SV><<<gy, M::NT, 0, c->stream>>>(c->T, c->tw, nc, (long)N * NH, NH, 0.0);
@@ -218,6 +219,7 @@ struct BatchPipe {
     a.uvb = p->uvb_flag;
     a.use_xe = p->include_spin_temp;
     a.nfield = c.F;
+    a.kzlive = N / 2;
     const int nb = nb_threshold();
     for (int s = 0; s < c.S; ++s) {
       const bool last = (s == c.S - 1);

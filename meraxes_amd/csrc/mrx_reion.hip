@@ -289,6 +289,17 @@ WinArgs win_args(double box_size, float R, int type, int apply)
   return w;
 }
 
+// Largest |n| component of a mode the sharp-k window of w can keep, capped at
+// N/2.  A mode with a component m > nmax has 0.413566994 |k| R > 1 + 1e-5
+// before rounding; the float evaluation of the window (relative error below
+// 1e-6) then still exceeds 1, so filter() zeroes it.
+int sharpk_nmax(const WinArgs& w, int N)
+{
+  const double kc = 1.0 / (0.413566994 * (double)w.R * (double)w.delta_k);
+  const double m = std::floor(kc * (1.0 + 1e-5)) + 1.0;
+  return (m >= (double)(N / 2)) ? N / 2 : (int)m;
+}
+
 // ----------------------------------------------------- templated pipelines
 template <int N>
 struct Pipe {
@@ -328,9 +339,34 @@ struct Pipe {
   }
 
   // U (+V) filtered at R -> T, then inverse y pass; T then holds the packed
-  // spectra of the filtered fields with x and y already transformed.
-  static int inverse_xy(mrx_ctx* ctx, const float2* U, const float2* V, float2* T, int nfield, const WinArgs& w)
+  // spectra of the filtered fields with x and y already transformed, in kz
+  // slots below kzlive (the slots above are zero and left unwritten: a
+  // sharp-k window keeps only a ball of modes, so only the tiles that hold
+  // part of it are read, transformed and written).
+  static int inverse_xy(mrx_ctx* ctx, const float2* U, const float2* V, float2* T, int nfield, const WinArgs& w,
+                        int& kzlive)
   {
+    kzlive = NH;
+    if (w.apply && w.type == MRX_FILTER_SHARPK) {
+      constexpr int TW = Cfg<N>::TW;
+      const int nmax = sharpk_nmax(w, N);
+      const int nkzb = std::min(NH / TW, nmax / TW + 1);
+      const int nky = std::min(N, 2 * nmax + 1);
+      {
+        PROF("ix_filter");
+        k_ix_filter<N, kWinSharpK><<<nky * nkzb, Cfg<N>::IX_NT, 0, ctx->stream>>>(
+          U, V, T, ctx->tw, nullptr, nc, (long)N * N, nfield, 0, 0, w, nmax, nkzb);
+      }
+      MRX_TRY(check_launch(ctx, "k_ix_filter (sharp-k)"));
+      {
+        PROF("colpass_y_inv");
+        k_colpass_live<N><<<dim3(N * nkzb, nfield), Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH,
+                                                                                    NH, nkzb, nmax);
+      }
+      MRX_TRY(check_launch(ctx, "k_colpass_live y inv"));
+      kzlive = nkzb * TW;
+      return MRX_OK;
+    }
     const float* Wt = nullptr;
     if (w.apply) {
       const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
@@ -346,9 +382,11 @@ struct Pipe {
     {
     PROF("ix_filter");
     if (Wt)
-      k_ix_filter<N, true><<<gc, Cfg<N>::IX_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
+      k_ix_filter<N, kWinTable><<<gc, Cfg<N>::IX_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield,
+                                                                       0, 0, w, N, NH / Cfg<N>::TW);
     else
-      k_ix_filter<N, false><<<gc, Cfg<N>::IX_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield, 0, 0);
+      k_ix_filter<N, kWinNone><<<gc, Cfg<N>::IX_NT, 0, ctx->stream>>>(U, V, T, ctx->tw, Wt, nc, (long)N * N, nfield,
+                                                                      0, 0, w, N, NH / Cfg<N>::TW);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter"));
     const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
@@ -357,11 +395,12 @@ struct Pipe {
     return check_launch(ctx, "k_colpass y inv");
   }
 
-  static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp, int pitch = N + 2)
+  static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp, int pitch = N + 2,
+                   int kzlive = NH)
   {
     const dim3 gz(N * N / Cfg<N>::RZ, nfield);
     PROF("c2r_z");
-    k_c2r_z<N><<<gz, kNT, 0, ctx->stream>>>(T, out, ctx->tw, nc, (long)N * N * pitch, clamp, pitch);
+    k_c2r_z<N><<<gz, kNT, 0, ctx->stream>>>(T, out, ctx->tw, nc, (long)N * N * pitch, clamp, pitch, kzlive);
     return check_launch(ctx, "k_c2r_z");
   }
 
@@ -508,7 +547,7 @@ struct HiiRun {
       for (int s = 0; s < S; ++s) {
         const bool last = (s == S - 1);
         const WinArgs w = win_args(box_size, (float)R[s], p->filter_type, last ? 0 : 1);
-        MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, nfield, w));
+        MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, nfield, w, a.kzlive));
         a.J21c = (1.0 + z) * (1.0 + z) / (4.0 * M_PI) * p->alpha_uv * kPlanck * 1e21 * R[s] * p->unit_length_cm *
                  p->nion_phot_per_bary / kProtonMass * p->unit_mass_g / pow(p->unit_length_cm, 3) / p->unit_time_s;
         a.M_mean = RtoM(p, R[s]);
@@ -548,8 +587,9 @@ struct FilteredField {
       MRX_TRY(ensure_buffers(ctx, N, 1));
       MRX_TRY(P::forward(ctx, in, 0, 1, true, ctx->U, ctx->V));
       const WinArgs w = win_args(box_size, (float)R, filter_type, apply);
-      MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
-      return P::c2r_z(ctx, ctx->T, out, 1, clamp);
+      int kzlive;
+      MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w, kzlive));
+      return P::c2r_z(ctx, ctx->T, out, 1, clamp, N + 2, kzlive);
     }
   };
 };
@@ -745,6 +785,10 @@ struct TsRun {
         MRX_TRY(ensure_buf(ctx, ctx->ts_work, ctx->ts_work_cap, np));
         MRX_TRY(ensure_buf(ctx, ctx->ts_shells, ctx->ts_shells_cap, (size_t)s->nsteps * ncell));
         HIP_TRY(hipMemcpyAsync(ctx->ts_work, sfr, sizeof(float) * np, hipMemcpyDeviceToDevice, ctx->stream));
+        // f_coll and <x_e> of the first shell (ComputeTs.c:413-475) read the
+        // input grids only: taken first, so the host builds the tables while
+        // the device filters the shells
+        if (!p->fixed_globals) MRX_TRY(fcoll(s->R[0], 1));
         const long half = (long)N * N * (N / 2 + 1);
         for (int ct = 0; ct < s->nsteps; ++ct) {
           for (int k = 0; k < s->n_ops[ct]; ++k) {
@@ -760,9 +804,9 @@ struct TsRun {
           // forward transform of an unchanged grid is reused
           if (ct == 0 || s->n_ops[ct] > 0) MRX_TRY(P::forward(ctx, ctx->ts_work, 0, 1, true, ctx->U, ctx->V, true));
           const WinArgs w = win_args(p->box_size, (float)s->R[ct], p->heating_filter_type, ct > 0 ? 1 : 0);
-          MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
-          MRX_TRY(P::c2r_z(ctx, ctx->T, ctx->ts_shells + (size_t)ct * ncell, 1, 3, N));
-          if (ct == 0 && !p->fixed_globals) MRX_TRY(fcoll(s->R[0], 1));
+          int kzlive;
+          MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w, kzlive));
+          MRX_TRY(P::c2r_z(ctx, ctx->T, ctx->ts_shells + (size_t)ct * ncell, 1, 3, N, kzlive));
         }
         if (p->fixed_globals) {
           collapse_fraction = p->fixed_collapse_fraction;
@@ -939,8 +983,9 @@ struct PowerSpec {
         ctx->U, ctx->V, dlam, a, a.volume / pow((double)N, 3));
       MRX_TRY(check_launch(ctx, "k_ps_adjoint_scale"));
       const WinArgs w = win_args(box_size_mpc, 1.0f, 0, 0);
-      MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
-      MRX_TRY(P::c2r_z(ctx, ctx->T, grad_out, 1, -1, N));
+      int kzlive;
+      MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w, kzlive));
+      MRX_TRY(P::c2r_z(ctx, ctx->T, grad_out, 1, -1, N, kzlive));
       HIP_TRY(hipStreamSynchronize(ctx->stream));
       return nb;
     }
@@ -1029,8 +1074,9 @@ struct SnapAdj {
         if (ct > 0) {  // shell 0 is the unfiltered grid
           MRX_TRY(P::forward(ctx, gshell, 0, 1, true, ctx->U, ctx->V, true));
           const WinArgs w = win_args(tp->box_size, (float)s->R[ct], tp->heating_filter_type, 1);
-          MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w));
-          MRX_TRY(P::c2r_z(ctx, ctx->T, tmp, 1, -1, N + 2));
+          int kzlive;
+          MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w, kzlive));
+          MRX_TRY(P::c2r_z(ctx, ctx->T, tmp, 1, -1, N + 2, kzlive));
           add = tmp;
         }
         k_axpy<<<(unsigned)((np + 255) / 256), 256, 0, ctx->stream>>>(g_sfr, add, (long)np);

@@ -91,22 +91,34 @@ struct WinArgs {
   int apply;      // 0: the unfiltered last step
 };
 
+// |k| of mode (nx, ny, nz) as filter() forms it (reionization.c:2116-2136).
+__device__ __forceinline__ float k_magnitude(int nx, int ny, int nz, int N, float dk)
+{
+  const float kx = kcomp(nx, N, dk);
+  const float ky = kcomp(ny, N, dk);
+  const float kz = (float)nz * dk;
+  return sqrtf(kx * kx + ky * ky + kz * kz);
+}
+
+// Sharp-k multiplier (reionization.c:2148-2152): 0 or 1.
+__device__ __forceinline__ float sharpk_window(float k_mag, float R)
+{
+  const float kR = (float)((double)(k_mag * R) * 0.413566994);
+  return (kR > 1.0f) ? 0.0f : 1.0f;
+}
+
 // Multiplier filter() applies to mode (nx, ny, nz); 1 where it leaves it alone.
 __device__ __forceinline__ float window(int nx, int ny, int nz, int N, const WinArgs& w)
 {
   if (!w.apply) return 1.0f;
-  const float kx = kcomp(nx, N, w.delta_k);
-  const float ky = kcomp(ny, N, w.delta_k);
-  const float kz = (float)nz * w.delta_k;
-  const float k_mag = sqrtf(kx * kx + ky * ky + kz * kz);
+  const float k_mag = k_magnitude(nx, ny, nz, N, w.delta_k);
   float kR = k_mag * w.R;
   switch (w.type) {
     case 0:
       if ((double)kR > 1e-4) return tophat_window(kR);
       return 1.0f;
     case 1:
-      kR = (float)((double)kR * 0.413566994);
-      return (kR > 1.0f) ? 0.0f : 1.0f;
+      return sharpk_window(k_mag, w.R);
     default: {
       // powf((float)M_E, y) = exp(y ln((float)M_E)), evaluated in double and
       // rounded once: the correctly rounded value libm's powf returns
@@ -217,17 +229,21 @@ __device__ __forceinline__ void c2r_untwist_fft(float2* s, const float2* tw, int
 }
 
 // Loads RZ packed rows, untwists them and runs the inverse length-N/2 FFT in
-// LDS; afterwards real cell e of the tile is lds_real<N>(s, e).
+// LDS; afterwards real cell e of the tile is lds_real<N>(s, e).  Slots kz >=
+// kzlive are read as zero (not loaded).
 template <int N>
 __device__ __forceinline__ void c2r_rows_lds(float2* s, const float2* tw, const float2* __restrict__ src, long row0,
-                                             int tid)
+                                             int tid, int kzlive)
 {
   using C = Cfg<N>;
   constexpr int NH = C::NH, ZE = C::ZE;
   constexpr int EPZ = ZE / kNT;
   float2 v[EPZ];
 #pragma unroll
-  for (int i = 0; i < EPZ; ++i) v[i] = src[row0 * NH + tid + i * kNT];
+  for (int i = 0; i < EPZ; ++i) {
+    const int e = tid + i * kNT;
+    v[i] = (e % NH < kzlive) ? src[row0 * NH + e] : make_float2(0.0f, 0.0f);
+  }
 #pragma unroll
   for (int i = 0; i < EPZ; ++i) s[lds_addr<true>(tid + i * kNT)] = v[i];
   __syncthreads();
@@ -260,11 +276,12 @@ __device__ __forceinline__ float clamp_kind(float v, int kind)
   }
 }
 
-// Plain C2R z pass: packed rows -> padded real rows, with an optional clamp.
+// Plain C2R z pass: packed rows -> padded real rows, with an optional clamp;
+// slots kz >= kzlive of the input are zero.
 template <int N>
 __global__ __launch_bounds__(kNT) void k_c2r_z(const float2* __restrict__ in, float* __restrict__ out,
                                               const float2* __restrict__ twN, long in_fstride, long out_fstride,
-                                              int clamp, int pitch)
+                                              int clamp, int pitch, int kzlive)
 {
   using C = Cfg<N>;
   constexpr int RZ = C::RZ, ZE = C::ZE;
@@ -273,7 +290,7 @@ __global__ __launch_bounds__(kNT) void k_c2r_z(const float2* __restrict__ in, fl
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += kNT) tw[i] = twN[i];
   const long row0 = (long)blockIdx.x * RZ;
-  c2r_rows_lds<N>(s, tw, in + blockIdx.y * in_fstride, row0, tid);
+  c2r_rows_lds<N>(s, tw, in + blockIdx.y * in_fstride, row0, tid, kzlive);
   float* o = out + blockIdx.y * out_fstride;
   for (int e = tid; e < RZ * pitch; e += kNT) {
     const int r = e / pitch, z = e % pitch;
@@ -359,21 +376,29 @@ __global__ __launch_bounds__(kNT) void k_window_table(float* __restrict__ Wt, Wi
     Wt[(long)NH1 * NH * NH1 + (long)b * NH1 + a] = v;
 }
 
+// Window modes of the inverse x pass.
+enum { kWinNone = 0, kWinTable = 1, kWinSharpK = 2 };
+
 // Inverse x pass with the window applied on load, for every field of the
 // ladder step.  U: unfiltered packed spectra (kz=0 slot = true kz=0 plane);
-// V: their Nyquist planes [ky][kx]; T: output packed spectra; Wt: the
-// radius's window table (WIN), unused for the unfiltered last step.  The
-// tile's window values are loaded once for all fields, and the next field's
-// tile is loaded while the current one is transformed.  blockIdx.y selects a
-// box of a batch (strides bstride for U/T, bvstride for V).
-template <int N, bool WIN, int TW = Cfg<N>::TW, int SV = 0>
+// V: their Nyquist planes [ky][kx]; T: output packed spectra.  WIN selects the
+// window: none (the unfiltered last step), the radius's table Wt, or the
+// sharp-k window evaluated in place (wa), whose modes with any |component| >
+// nmax are zero: then only the tiles with |ky| <= nmax and kz block < nkzb are
+// launched (grid (min(N, 2 nmax + 1)) x nkzb), zero modes are stored as 0
+// without being read (filter() assigns 0 to them), and T's kz blocks >= nkzb
+// and rows |ky| > nmax are left unwritten for the passes that follow to read
+// as zero.  The tile's window values are loaded once for all fields, and the
+// next field's tile is loaded while the current one is transformed.
+// blockIdx.y selects a box of a batch (strides bstride for U/T, bvstride for V).
+template <int N, int WIN, int TW = Cfg<N>::TW, int SV = 0>
 __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const float2* __restrict__ U,
                                                                  const float2* __restrict__ V,
                                                                  float2* __restrict__ T,
                                                                  const float2* __restrict__ twN,
                                                                  const float* __restrict__ Wt, long fstride,
                                                                  long vstride, int nfield, long bstride,
-                                                                 long bvstride)
+                                                                 long bvstride, WinArgs wa, int nmax, int nkzb)
 {
   using M = TileMap<N, TW, kNT>;  // 256 threads: the window values stay in registers
   constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
@@ -383,21 +408,40 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   const int c = tid % TW, n0 = tid / TW;
-  const int ky = blockIdx.x / NKZB, b = blockIdx.x % NKZB;
+  int ky, b;
+  if constexpr (WIN == kWinSharpK) {
+    const int j = blockIdx.x / nkzb, nky = min(N, 2 * nmax + 1);
+    b = blockIdx.x % nkzb;
+    ky = (j <= nmax) ? j : N - (nky - j);
+  } else {
+    ky = blockIdx.x / NKZB;
+    b = blockIdx.x % NKZB;
+  }
   const int aky = ky <= NH ? ky : N - ky;
   U += blockIdx.y * bstride;
   T += blockIdx.y * bstride;
   V += blockIdx.y * bvstride;
   const long base = (long)ky * NH + (long)b * TW + (unsigned)n0 * rs + c;
   float2 r[EPT];
-  {
+  float w[WIN != kWinNone ? EPT : 1];
+  if constexpr (WIN == kWinSharpK) {
+    // window first, then only the live modes are read
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int n = n0 + i * RPI;
+      const int an = n <= NH ? n : N - n;
+      w[i] = (an <= nmax) ? sharpk_window(k_magnitude(an, aky, b * TW + c, N, wa.delta_k), wa.R) : 0.0f;
+    }
+    const float2* u = U + base;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) r[i] = (w[i] != 0.0f) ? u[(unsigned)(i * RPI) * rs] : make_float2(0.0f, 0.0f);
+  } else {
     const float2* u = U + base;
 #pragma unroll
     for (int i = 0; i < EPT; ++i) r[i] = u[(unsigned)(i * RPI) * rs];
   }
   // this tile's block of the window table (k_window_table's layout)
-  float w[WIN ? EPT : 1];
-  if constexpr (WIN) {
+  if constexpr (WIN == kWinTable) {
     const float* wr = Wt + ((long)aky * NKZB + b) * NH1 * TW + c;
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
@@ -407,17 +451,25 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
     }
   }
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
+  // the Nyquist plane's window is nonzero somewhere on this tile row
+  const bool nyq = (b == 0) && (WIN != kWinSharpK || (nmax >= NH && aky <= nmax));
   for (int f = 0; f < nfield; ++f) {
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = WIN ? cscale(r[i], w[i]) : r[i];
-    if (b == 0) {
+    for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = (WIN == kWinTable) ? cscale(r[i], w[i]) : r[i];
+    if (nyq) {
       // the kz=0 slot carries A*W0 + i*(B*WN), B the Nyquist plane
       __syncthreads();
       const float2* vp = V + f * vstride + (long)ky * N;
       for (int n = tid; n < N; n += NT) {
         const int an = n <= NH ? n : N - n;
-        const float wn = WIN ? Wt[(long)NH1 * NH * NH1 + (long)aky * NH1 + an] : 1.0f;
-        const float2 nb = cscale(vp[n], wn);
+        float2 nb;
+        if constexpr (WIN == kWinSharpK) {
+          if (an > nmax || sharpk_window(k_magnitude(an, aky, NH, N, wa.delta_k), wa.R) == 0.0f) continue;
+          nb = vp[n];
+        } else {
+          const float wn = (WIN == kWinTable) ? Wt[(long)NH1 * NH * NH1 + (long)aky * NH1 + an] : 1.0f;
+          nb = cscale(vp[n], wn);
+        }
         const float2 v = s[n * SP];
         s[n * SP] = make_float2(v.x - nb.y, v.y + nb.x);
       }
@@ -426,7 +478,12 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
     if (f + 1 < nfield) {
       const float2* u = U + (f + 1) * fstride + base;
 #pragma unroll
-      for (int i = 0; i < EPT; ++i) r[i] = u[(unsigned)(i * RPI) * rs];
+      for (int i = 0; i < EPT; ++i) {
+        if constexpr (WIN == kWinSharpK)
+          r[i] = (w[i] != 0.0f) ? u[(unsigned)(i * RPI) * rs] : make_float2(0.0f, 0.0f);
+        else
+          r[i] = u[(unsigned)(i * RPI) * rs];
+      }
     }
     fft_lds<N, TW, NT, true, SP, 1, true, 1, false, SV>(s, tw, tid);
     float2* t = T + f * fstride + base;
@@ -434,6 +491,42 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
     for (int i = 0; i < EPT; ++i) t[(unsigned)(i * RPI) * rs] = s[(n0 + i * RPI) * SP + c];
     __syncthreads();
   }
+}
+
+// Inverse pass over the live tiles of a sharp-k filtered spectrum left by
+// k_ix_filter<kWinSharpK>: kz blocks below nkzb only (grid N x nkzb per
+// field, tile t = line t / nkzb, kz block t % nkzb), and rows n with
+// min(n, N - n) > nlive read as zero.  Otherwise as k_colpass<N, true>.
+template <int N, int TW = Cfg<N>::TW>
+__global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass_live(float2* __restrict__ data,
+                                                                    const float2* __restrict__ twN, long fstride,
+                                                                    long astride, long rstride, int nkzb,
+                                                                    int nlive)
+{
+  using M = TileMap<N, TW>;
+  constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT;
+  constexpr int NH = N / 2;
+  __shared__ float2 s[N * SP];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  const int c = tid % TW, n0 = tid / TW;
+  const int a = blockIdx.x / nkzb, b = blockIdx.x % nkzb;
+  const unsigned rs = (unsigned)rstride;
+  float2* g = data + blockIdx.y * fstride + (long)a * astride + (long)b * TW + (unsigned)n0 * rs + c;
+  float2 r[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int n = n0 + i * RPI;
+    const int an = n <= NH ? n : N - n;
+    r[i] = (an <= nlive) ? g[(unsigned)(i * RPI) * rs] : make_float2(0.0f, 0.0f);
+  }
+  for (int i = tid; i < N; i += NT) tw[i] = twN[i];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = r[i];
+  __syncthreads();
+  fft_lds<N, TW, NT, true, SP, 1, true, 1, false, 0>(s, tw, tid);
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) g[(unsigned)(i * RPI) * rs] = s[(n0 + i * RPI) * SP + c];
 }
 
 // After the forward x pass the kz=0 slot holds P = A + iB (A, B the kz=0 and
@@ -554,6 +647,7 @@ struct StepArgs {
   int uvb;
   int use_xe;
   int nfield;
+  int kzlive;           // spectrum slots kz >= kzlive are zero (sharp-k)
 };
 
 // One ladder step of _find_HII_bubbles (find_HII_bubbles.c:255-443) for RZ
@@ -607,7 +701,7 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
       const int e = tid + i * NT, f = e / ZE, k = e % ZE;
-      v[i] = T[f * fstride + row0 * NH + k];
+      v[i] = (k % NH < a.kzlive) ? T[f * fstride + row0 * NH + k] : make_float2(0.0f, 0.0f);
     }
     for (int i = tid; i < N; i += NT) tw[i] = twN[i];
 #pragma unroll

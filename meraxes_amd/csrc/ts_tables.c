@@ -19,9 +19,11 @@
 
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "gk61_rule.h"
 
@@ -901,18 +903,92 @@ int mrx_ts_plan(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p)
   return MRX_OK;
 }
 
+/* The shells' tables are independent of one another: phase 0 computes each
+ * shell's lower frequency limit and raw Ly-n sum, phase 1 the 3 x 14
+ * frequency integrals of every shell; work items are handed out one at a
+ * time to up to 16 host threads.  Each item is computed exactly as the serial
+ * loop would, so the tables do not depend on the thread count. */
+typedef struct {
+  mrx_ts_snap* s;
+  const mrx_ts_data* d;
+  const mrx_ts_params* p;
+  double* lower;
+  int phase, nitems;
+  int next; /* next unclaimed work item */
+} tab_job;
+
+static void tab_item(tab_job* j, int it)
+{
+  mrx_ts_snap* s = j->s;
+  const mrx_ts_data* d = j->d;
+  const mrx_ts_params* p = j->p;
+  const double zp = s->zp;
+  if (j->phase == 0) {
+    const int ct = it;
+    const double prev_zpp = (ct == 0) ? zp : s->zpp_edge[ct - 1];
+    const double zpp = (s->zpp_edge[ct] + prev_zpp) * 0.5;
+    j->lower[ct] = fmax(mrx_ts_nu_tau_one(d, p, zp, zpp, s->x_e_ave, s->x_e_ave),
+                        p->nu_xray_gal_threshold * K_NU_over_EV);
+    double sum = 0;
+    for (int n_ct = MRX_NSPEC_MAX; n_ct >= 2; n_ct--) {
+      if (zpp > zmax((float)zp, n_ct)) continue;
+      const double nuprime = nu_n(n_ct) * (1 + zpp) / (1.0 + zp);
+      sum += frecycle(n_ct) * mrx_ts_spectral_emissivity(d, nuprime);
+    }
+    s->sum_lyn[ct] = sum;
+  } else {
+    const int ct = it / (3 * MRX_X_INT_NXHII), x = (it / 3) % MRX_X_INT_NXHII, kind = it % 3;
+    const double v = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], kind);
+    if (kind == 0)
+      s->freq_heat[x][ct] = v;
+    else if (kind == 1)
+      s->freq_ion[x][ct] = v;
+    else
+      s->freq_lya[x][ct] = v;
+  }
+}
+
+static void* tab_worker(void* vp)
+{
+  tab_job* j = (tab_job*)vp;
+  for (;;) {
+    const int it = __atomic_fetch_add(&j->next, 1, __ATOMIC_RELAXED);
+    if (it >= j->nitems) return NULL;
+    tab_item(j, it);
+  }
+}
+
+static void tab_run(tab_job* j)
+{
+  enum { MAXT = 16 };
+  long nt = sysconf(_SC_NPROCESSORS_ONLN);
+  if (nt > MAXT) nt = MAXT;
+  if (nt > j->nitems) nt = j->nitems;
+  pthread_t th[MAXT];
+  int started = 0;
+  for (int t = 1; t < nt; t++)
+    if (pthread_create(&th[started], NULL, tab_worker, j) == 0) started++;
+  tab_worker(j); /* the calling thread works too, and alone if no thread started */
+  for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
+}
+
 int mrx_ts_build_tables(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p)
 {
   if (s->homogeneous) return MRX_OK;
   const double zp = s->zp;
-  const double ReionEfficiency = p->reion_efficiency;
   const double collapse_fraction = s->collapse_fraction;
-  const double x_e_ave = s->x_e_ave;
   if (p->stored_fcoll) p->stored_fcoll[p->snapshot] = collapse_fraction;
   s->no_light = (collapse_fraction > 0.0) ? 0 : 1;
   int first_radii = 1, first_zero = 1;
   const int n_pts_radii = 1000;
   double weight = s->final_weight; /* the shell loop's last weight (one variable in the reference) */
+  double lower[MRX_TS_MAX_STEPS];
+  tab_job job = {s, d, p, lower, 0, s->nsteps, 0};
+  tab_run(&job);
+  job.phase = 1;
+  job.nitems = s->nsteps * MRX_X_INT_NXHII * 3;
+  job.next = 0;
+  tab_run(&job);
   for (int ct = 0; ct < s->nsteps; ct++) {
     double prev_zpp, prev_R;
     if (ct == 0) {
@@ -921,24 +997,6 @@ int mrx_ts_build_tables(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_param
     } else {
       prev_zpp = s->zpp_edge[ct - 1];
       prev_R = s->R[ct - 1];
-    }
-    const double zpp = (s->zpp_edge[ct] + prev_zpp) * 0.5;
-    double filling_factor_of_HI_zp = 1. - ReionEfficiency * collapse_fraction / (1.0 - x_e_ave);
-    const double lower_int_limit = fmax(mrx_ts_nu_tau_one(d, p, zp, zpp, x_e_ave, x_e_ave),
-                                        p->nu_xray_gal_threshold * K_NU_over_EV);
-    if (filling_factor_of_HI_zp < 0) filling_factor_of_HI_zp = 0;
-    (void)filling_factor_of_HI_zp;
-    for (int x = 0; x < MRX_X_INT_NXHII; x++) {
-      const double xe = d->xi_XHII[x];
-      s->freq_heat[x][ct] = mrx_ts_integrate_over_nu(d, p, zp, xe, lower_int_limit, 0);
-      s->freq_ion[x][ct] = mrx_ts_integrate_over_nu(d, p, zp, xe, lower_int_limit, 1);
-      s->freq_lya[x][ct] = mrx_ts_integrate_over_nu(d, p, zp, xe, lower_int_limit, 2);
-    }
-    s->sum_lyn[ct] = 0;
-    for (int n_ct = MRX_NSPEC_MAX; n_ct >= 2; n_ct--) {
-      if (zpp > zmax((float)zp, n_ct)) continue;
-      const double nuprime = nu_n(n_ct) * (1 + zpp) / (1.0 + zp);
-      s->sum_lyn[ct] += frecycle(n_ct) * mrx_ts_spectral_emissivity(d, nuprime);
     }
     /* smooth the kink where the Ly-n contribution first vanishes (ComputeTs.c:625-663) */
     if (ct > 2 && s->sum_lyn[ct] == 0.0 && s->sum_lyn[ct - 1] > 0. && first_radii) {
