@@ -227,6 +227,7 @@ struct BatchPipe {
       MRX_TRY(inverse_xy(ctx, c, last ? nullptr : &w, c.variant));
       a.M_mean = RtoM(p, c.R[s]);
       a.R_cubed = c.R[s] * c.R[s] * c.R[s];
+      a.kfc = (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume / a.M_mean;
       a.step = s;
       a.last = last ? 1 : 0;
       constexpr int NT = ThrCfg<N>::NT;

@@ -552,6 +552,7 @@ struct HiiRun {
                  p->nion_phot_per_bary / kProtonMass * p->unit_mass_g / pow(p->unit_length_cm, 3) / p->unit_time_s;
         a.M_mean = RtoM(p, R[s]);
         a.R_cubed = R[s] * R[s] * R[s];
+        a.kfc = (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume / a.M_mean;
         a.step = s;
         a.last = last ? 1 : 0;
         MRX_TRY(P::threshold(ctx, deltax, out[0], out[1], out[2], out[3], out[4], a));

@@ -369,11 +369,17 @@ __global__ __launch_bounds__(kNT) void k_window_table(float* __restrict__ Wt, Wi
   const int c = (int)(idx % NH1);
   const long ab = idx / NH1;
   const int b = (int)(ab % NH1), a = (int)(ab / NH1);
+  // kx^2 + ky^2 is commutative in float, so W(a, b, c) = W(b, a, c): the
+  // threads with a <= b evaluate the window once for both
+  if (a > b) return;
   const float v = window(a, b, c, N, w);
-  if (c < NH)
+  if (c < NH) {
     Wt[(((long)b * NKZB + c / TW) * NH1 + a) * TW + c % TW] = v;
-  else
+    Wt[(((long)a * NKZB + c / TW) * NH1 + b) * TW + c % TW] = v;
+  } else {
     Wt[(long)NH1 * NH * NH1 + (long)b * NH1 + a] = v;
+    Wt[(long)NH1 * NH * NH1 + (long)a * NH1 + b] = v;
+  }
 }
 
 // Window modes of the inverse x pass.
@@ -642,6 +648,7 @@ struct StepArgs {
   double eff;           // ReionEfficiency
   double redshift;
   double bias;          // ReionGammaHaloBias
+  double kfc;           // (4/3) pi R^3 / (pixel_volume M_mean): f_coll = stars * kfc / (1 + delta)
   int step;             // ladder index (0 = largest R)
   int last;             // unfiltered final step
   int uvb;
@@ -741,25 +748,39 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
     uint32_t st_new = st_old;
     float xh4[4], rb4[4], j4[4];
     int zcheck4 = 0;
-    float jaux4[4];
+    float jaux4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int i = 4 * j + q;
       const long ir = ir0 + q;
       const double density_over_mean = 1.0 + (double)val[0][i];
-      const double f_coll_stars =
-        (double)val[1][i] / (a.M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume;
-      const double weighted_sfr_density = (double)val[2][i] / a.pixel_volume;
       const double neutral_fraction = a.use_xe ? 1.0 - (double)val[3][i] : 1.0;
-      const float J_21_aux = a.uvb ? (float)(weighted_sfr_density * a.J21c) : 0.0f;
-      jaux4[q] = J_21_aux;
-      const bool cross = f_coll_stars * a.eff > neutral_fraction * (1. + 0.0);
+      // the reference's f_coll (four roundings of its exact value), evaluated
+      // where needed: the last step's partial xH and near-threshold cells
+      auto f_coll = [&]() {
+        return (double)val[1][i] / (a.M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * a.R_cubed /
+               a.pixel_volume;
+      };
+      // the test f_coll*eff > n_f as stars*kfc*eff vs n_f*(1+delta), both
+      // positive sides within 1e-14 of their exact values: a relative gap
+      // above 1e-12 fixes the reference's outcome, closer cells take its
+      // own expression
+      const double lhs = (double)val[1][i] * a.kfc * a.eff, rhs = neutral_fraction * density_over_mean;
+      bool cross;
+      if (lhs > rhs * (1.0 + 1e-12))
+        cross = true;
+      else if (lhs < rhs * (1.0 - 1e-12))
+        cross = false;
+      else
+        cross = f_coll() * a.eff > neutral_fraction * (1. + 0.0);
+      // J_21 of the cell at this radius (:317-320, :331-337)
+      auto j21_aux = [&]() { return a.uvb ? (float)((double)val[2][i] / a.pixel_volume * a.J21c) : 0.0f; };
       const uint32_t st = (st_old >> (8 * q)) & 0xFFu;
       const bool never = (st == 0xFFu);
       bool zcheck = false;
       if (cross) {
         if (never) {
-          if (a.uvb) J21[ir] = J_21_aux;
+          if (a.uvb) J21[ir] = j21_aux();
           zcheck = true;
         }
         st_new = (st_new & ~(0xFFu << (8 * q))) | ((uint32_t)a.step << (8 * q));
@@ -770,7 +791,7 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
         if (ion) {
           xh = 0.0f;
         } else {
-          xh = (float)(neutral_fraction - f_coll_stars * a.eff);
+          xh = (float)(neutral_fraction - f_coll() * a.eff);
           if (xh < 0.f)
             xh = 0.f;
           else if (xh > 1.0f)
@@ -781,6 +802,7 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
         rb4[q] = ion ? Rtab[cross ? a.step : (int)st] : 0.0f;
         j4[q] = ion ? 1.0f : 0.0f;  // marks the cells whose J_21 is kept
       }
+      if (zcheck) jaux4[q] = j21_aux();
       zcheck4 |= (zcheck ? 1 : 0) << q;
     }
     if (st_new != st_old || a.step == 0) *reinterpret_cast<uint32_t*>(state + ir0) = st_new;

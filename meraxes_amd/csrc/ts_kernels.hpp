@@ -144,16 +144,22 @@ __device__ __forceinline__ R get_Ts_t(const TsDev& t, R dl, R TK, R xe, R Jalpha
     R TSold(0.0), xa(0.0);
     TS = R(Trad);
     const R tgp = t.TGP * (1.0 + dl) * (1.0 - xe);
+    // the terms of the iteration that do not involve TS, evaluated once (the
+    // same operations in the same order as the reference's loop body)
+    const R xi = pow(1.0e-7 * tgp / TK / TK, 1.0 / 3.0);
+    const R S0 = 1.0 - 0.0631789 / TK + 0.115995 / TK / TK;
+    const R Sden = 1.0 + 2.98394 * xi + 1.53583 * xi * xi + 3.85289 * xi * xi * xi;
+    const R invTK = 1.0 / TK, ceTK = 0.405535 / TK, xcTK = xc / TK;
+    const double invTrad = 1.0 / Trad;
     while (fabs(pval(TS) - pval(TSold)) / pval(TS) > 1.0e-3) {
       TSold = TS;
-      const R xi = pow(1.0e-7 * tgp / TK / TK, 1.0 / 3.0);
-      R S = 1.0 - 0.0631789 / TK + 0.115995 / TK / TK - 0.401403 / TS / TK;
+      R S = S0 - 0.401403 / TS / TK;
       S += 0.336463 / TS / TK / TK;
-      S /= 1.0 + 2.98394 * xi + 1.53583 * xi * xi + 3.85289 * xi * xi * xi;
+      S /= Sden;
       xa = t.XA * S * Jalpha;
-      const R ce = 1.0 / TK + 0.405535 / TK * (1.0 / TS - 1.0 / TK);
+      const R ce = invTK + ceTK * (1.0 / TS - invTK);
       const R Tceff = 1.0 / ce;
-      const R TSinv = (1.0 / Trad + xa / Tceff + xc / TK) / (1.0 + xa + xc);
+      const R TSinv = (invTrad + xa / Tceff + xcTK) / (1.0 + xa + xc);
       TS = 1.0 / TSinv;
     }
     *xa_out = xa;
@@ -180,9 +186,13 @@ template <class R>
 __device__ __forceinline__ R alpha_A_t(R T)
 {
   const R logT = log(T / (double)1.1604505e4);
-  return exp(-28.6130338 - 0.72411256 * logT - 2.02604473e-2 * pow(logT, 2) - 2.38086188e-3 * pow(logT, 3) -
-             3.21260521e-4 * pow(logT, 4) - 1.42150291e-5 * pow(logT, 5) + 4.98910892e-6 * pow(logT, 6) +
-             5.75561414e-7 * pow(logT, 7) - 1.85676704e-8 * pow(logT, 8) - 3.07113524e-9 * pow(logT, 9));
+  // the reference's pow(logT, k) as products (each within a few ulp of pow,
+  // which is itself not correctly rounded); the sum keeps its order
+  const R p2 = logT * logT, p3 = p2 * logT, p4 = p2 * p2, p5 = p4 * logT, p6 = p3 * p3, p7 = p6 * logT,
+          p8 = p4 * p4, p9 = p8 * logT;
+  return exp(-28.6130338 - 0.72411256 * logT - 2.02604473e-2 * p2 - 2.38086188e-3 * p3 - 3.21260521e-4 * p4 -
+             1.42150291e-5 * p5 + 4.98910892e-6 * p6 + 5.75561414e-7 * p7 - 1.85676704e-8 * p8 -
+             3.07113524e-9 * p9);
 }
 
 __device__ __forceinline__ double dev_alpha_A(double T) { return alpha_A_t<double>(T); }
