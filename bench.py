@@ -11,12 +11,18 @@ The metric is cells*radii per second (N^3 * bubble radii / step time).
 
 Besides the headline line it reports, from HIP events recorded around every
 kernel launch on the stream the kernels run on, the dominant kernel's average
-launch time and its achieved HBM bandwidth against the MI355X peak, the
-whole-path fraction of peak on the 3-pass byte model, and a CPU baseline: the
+launch time and its achieved HBM bandwidth against the MI355X peak (and its
+PMC-measured bytes per launch where a summary under profiles/ holds them), the
+per-kernel table, the device time per stage, the path's fraction of peak on
+the 3-pass byte model, a 1 GiB device-copy ceiling, and a CPU baseline: the
 oracle (the C restatement of the reference's FFTW-float path) on the host cores
-over a bounded sample of the same workload.
+over a bounded sample of the same workload.  Unless --no-extras is given the
+snapshot run also times the hand-written adjoint against its forward at 256^3
+and 512^3, and the batched bubble finder (96 uneven boxes, one graph launch)
+against the per-box loop.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512]
+           [--workload snapshot|bubbles|batched] [--no-extras] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -51,8 +57,10 @@ def kernel_bytes(name, N, F):
     """
     if name.startswith("ts_"):
         name, F = name[3:], 1
-        if name == "c2r_z":  # shell: spectrum in, float shell out
-            return 8 * N * N * (N // 2) + 4 * N**3
+        if name in ("ix_filter", "colpass_y_inv", "c2r_z"):
+            # sharp-k shells: only the tiles holding the ball of kept modes are
+            # read (and, in the x/y passes, written), so the bytes depend on R
+            return None
         if name == "cell":
             name = "ts_cell"
     spec = 8 * N * N * (N // 2)
@@ -466,9 +474,14 @@ def main():
                          "GB_s": (kb * n / (ms / 1e3) / 1e9) if kb else None,
                          "frac": (kb * n / (ms / 1e3) / HBM_PEAK) if kb else None}
     Bc = 8 * N * N * (N // 2 + 1)
-    mb = model_bytes_hii(N, F, S, True) + 16 * N**3
+    mb_hii = model_bytes_hii(N, F, S, True)
+    mb = mb_hii + 16 * N**3
     if snapshot:
         mb += 40 * (12 * Bc + 8 * N**3) + 184 * N**3  # the 3-pass model of the Ts shells and per-cell pass
+    # device time per stage (sums of the kernels' event timings)
+    stage = {"ts": sum(v[0] for k, v in prof.items() if k.startswith("ts_")) / args.steps,
+             "dTb": sum(v[0] for k, v in prof.items() if k == "brightness_temp") / args.steps}
+    stage["bubbles"] = sum(v[0] for v in prof.values()) / args.steps - stage["ts"] - stage["dTb"]
     cells_radii = N**3 * S
     out = {
         "metric": ("cells*radii/s of one snapshot (Ts + find_HII_bubbles + dTb) at 512^3, plain path" if snapshot
@@ -490,6 +503,14 @@ def main():
                    "fields": F, "ts_shells": 40 if snapshot else 0, "parallelism": "single GPU"},
         "model_hbm_frac": mb / dt / HBM_PEAK,
         "model_bytes": mb,
+        "model_note": ("bytes of the 3-pass model of the reference's structure (separate filter, clamp and "
+                       "threshold traffic, float state grids, dense shell transforms) over the step's wall time; "
+                       "this path fuses the filter, clamp and threshold into the transform passes, keeps one byte "
+                       "of ladder state per cell and touches only the live modes of the sharp-k shells, so it "
+                       "moves fewer bytes than the model: the kernels' GB_s (their own algorithmic bytes) and "
+                       "roofline.traffic (PMC) are the HBM figures"),
+        "bubbles_model_frac": mb_hii / (stage["bubbles"] / 1e3) / HBM_PEAK,
+        "stage_device_ms": stage,
         "roofline": roof,
         "kernels": per_kernel,
     }
