@@ -18,8 +18,8 @@ the 3-pass byte model, a 1 GiB device-copy ceiling, and a CPU baseline: the
 oracle (the C restatement of the reference's FFTW-float path) on the host cores
 over a bounded sample of the same workload.  Unless --no-extras is given the
 snapshot run also times the hand-written adjoint against its forward at 256^3
-and 512^3, and the batched bubble finder (96 uneven boxes, one graph launch)
-against the per-box loop.
+and 512^3, the batched bubble finder (96 uneven boxes, one graph launch)
+against the per-box loop, and the galaxy gridding (2^24 galaxies onto 512^3).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512]
            [--workload snapshot|bubbles|batched] [--no-extras] [--no-cpu-baseline]
@@ -240,6 +240,60 @@ def bench_adjoint(ctx, N, steps=2):
 
 
 BATCH_SIZES = (64, 128, 192)
+def bench_gridding(ctx, N=512, n_gal=1 << 24, reps=3):
+    """Galaxy gridding (construct_baryon_grids) of 2^24 galaxies onto 512^3 grids.
+
+    Spin temperature on (stars, weighted_sfr, sfr and an 8-slot SFR history
+    ring), black-hole feedback on; galaxy arrays and grids resident in HBM.
+    Positions uniform with a tenth of the galaxies in 16 clumps.
+    """
+    import numpy as np
+    import torch
+
+    from meraxes_amd import params as P
+    from meraxes_amd import reion
+
+    L = 400.0 * N / 512
+    g = torch.Generator(device="cuda").manual_seed(11)
+    dev = torch.device("cuda:0")
+    pos = torch.rand((n_gal, 3), generator=g, device=dev) * L
+    k = n_gal // 10
+    centres = torch.rand((16, 3), generator=g, device=dev) * L
+    pick = torch.randint(0, 16, (k,), generator=g, device=dev)
+    pos[:k] = torch.remainder(centres[pick] + torch.randn((k, 3), generator=g, device=dev) * (L / 400), L)
+    pos = pos.float().clamp_(0, float(np.nextafter(np.float32(L), 0))).contiguous()
+    type_ = torch.randint(0, 4, (n_gal,), generator=g, device=dev, dtype=torch.int32)
+    logn = (lambda m, s: torch.exp(m + s * torch.randn(n_gal, generator=g, device=dev, dtype=torch.float64)))
+    fesc, gross, bhm, ebhm = logn(-3, 2), logn(-2, 2), logn(-4, 1.5), logn(-6, 2)
+    p = P.grid_params(N, L, 8.4, include_spin_temp=1, n_store_sfr=8, bh_feedback=1)
+    shape = (N, N, 2 * (N // 2 + 1))
+    stars, wsfr, sfr = (torch.empty(shape, device=dev) for _ in range(3))
+    hist = torch.zeros((8,) + shape, device=dev)
+    run = (lambda: reion.construct_baryon_grids(ctx, p, pos, type_, fesc, gross, bhm, ebhm, stars=stars,
+                                                 weighted_sfr=wsfr, sfr=sfr, sfr_histories=hist))
+    r = run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    np_ = 2 * N * N * (N // 2 + 1)
+    # per call: galaxy arrays (12 + 4 + 4 * 8 B) read for the keys and the sums,
+    # keys/values written, sorted and read; grids zeroed and written; the
+    # history ring moved by one slot (7 of 8 slots read and written)
+    grid_bytes = 4 * np_ * (3 + 3 + 2 * 7 + 1)
+    log(f"[bench] gridding {n_gal} galaxies onto {N}^3: {dt * 1e3:.2f} ms ({n_gal / dt:.3e} galaxies/s)")
+    del pos, type_, fesc, gross, bhm, ebhm, stars, wsfr, sfr, hist
+    torch.cuda.empty_cache()
+    return {"metric": "galaxies/s of construct_baryon_grids (stars, weighted_sfr, sfr, 8-slot history)",
+            "value": n_gal / dt, "ms_per_call": dt * 1e3, "n_galaxies": n_gal, "grid": N,
+            "n_gridded": r["n_gridded"], "grid_bytes_per_call": grid_bytes,
+            "grid_GB_s": grid_bytes / dt / 1e9,
+            "config": f"{n_gal} galaxies, {N}^3, {L:g} Mpc/h, spin temperature and black-hole feedback on, "
+                      f"a tenth in 16 clumps, synthetic (torch generator)"}
+
+
 BATCH_PER_SIZE = 32
 
 
@@ -346,8 +400,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["snapshot", "bubbles", "batched"], default="snapshot")
-    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched parts of the snapshot run")
+    ap.add_argument("--workload", choices=["snapshot", "bubbles", "batched", "gridding"], default="snapshot")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the adjoint, batched and gridding parts of the snapshot run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (scale of the PMC byte counters)")
     args = ap.parse_args()
@@ -381,6 +436,17 @@ def main():
                "warmup": args.warmup, "ms_per_step": b["ms_per_launch"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": b["config"],
                "batched": b}
+        print(json.dumps(out), flush=True)
+        return
+
+    if args.workload == "gridding":
+        ctx = Context(0)
+        b = bench_gridding(ctx, N=args.grid, reps=max(args.steps, 1))
+        ctx.close()
+        out = {"metric": b["metric"], "value": b["value"], "unit": "galaxies/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": 1, "ms_per_step": b["ms_per_call"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": {"workload": b["config"]},
+               "gridding": b}
         print(json.dumps(out), flush=True)
         return
 
@@ -521,6 +587,7 @@ def main():
         torch.cuda.empty_cache()
         out["adjoint"] = {str(n): bench_adjoint(ctx, n) for n in (256, N)}
         out["batched"] = bench_batched(ctx, 2, 1)
+        out["gridding"] = bench_gridding(ctx)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(snapshot)

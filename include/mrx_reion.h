@@ -329,9 +329,68 @@ int mrx_hii_batch_run(mrx_hii_batch* batch, mrx_hii_globals* out);
 int mrx_hii_batch_describe(const mrx_hii_batch* batch, char* buf, int len);
 int mrx_hii_batch_free(mrx_hii_batch* batch);
 
+/*
+ * Galaxy gridding: construct_baryon_grids (reionization.c:1353-1615) for one
+ * process holding the whole box.  Each galaxy that is not dead (type <= 2) is
+ * deposited on its nearest grid point, pos_to_ngp (misc_tools.c:163-173):
+ * i = nearbyint(pos / box_size * N), wrapped to 0 when > N-1; a negative index
+ * is an error (the reference asserts).  Per cell, the contributions are summed
+ * in galaxy order into a float exactly as the reference's `buffer[ind] += v`
+ * (v double, the sum rounded to float after each addition):
+ *   stars        += fesc_weighted_gsm  (+ effective_bhm when bh_feedback and
+ *                                        black_hole_mass >= bh_mass_limit_reion)
+ *   weighted_sfr += fesc_weighted_gsm  (+ effective_bhm * alpha_uv_bh / alpha_uv, same condition)
+ *   sfr          += gross_stellar_mass (include_spin_temp only)
+ * then written to the padded grids (zero elsewhere): stars clamped at 0;
+ * weighted_sfr and sfr as (float)(v / sfr_timescale) where v > 0, else 0,
+ * with sfr_timescale = reion_sfr_timescale * hubble_time(snapshot)
+ * (virial_properties.c:79-93).  With include_spin_temp the SFR history ring
+ * (n_store_sfr padded grids) is shifted by one slot first and slot 0 receives
+ * the new sfr grid at every real cell (its padding is left as it was, as in the
+ * reference).
+ */
+typedef struct mrx_grid_params {
+  int grid_dim;             /* ReionGridDim (N)                                              */
+  int include_spin_temp;    /* Flag_IncludeSpinTemp: also the sfr grid and the history ring  */
+  int bh_feedback;          /* physics.Flag_BHFeedback                                       */
+  int n_store_sfr;          /* NstoreSnapshots_SFR: slots of sfr_histories                   */
+  int pointers_on_device;   /* 1: the galaxy arrays and grids are device memory             */
+  int pad_;                 /* unused, 0                                                     */
+  double box_size;          /* BoxSize [Mpc/h]                                               */
+  double reion_sfr_timescale; /* ReionSfrTimescale                                           */
+  double hubble;            /* run_globals.Hubble [internal units]                           */
+  double omega_m;           /* OmegaM                                                        */
+  double omega_k;           /* OmegaK                                                        */
+  double omega_lambda;      /* OmegaLambda                                                   */
+  double redshift;          /* ZZ[snapshot]                                                  */
+  double bh_mass_limit_reion; /* physics.BlackHoleMassLimitReion                             */
+  double alpha_uv_bh;       /* physics.ReionAlphaUVBH                                        */
+  double alpha_uv;          /* physics.ReionAlphaUV                                          */
+} mrx_grid_params;
+
+typedef struct mrx_grid_stats {
+  long n_gridded;           /* galaxies deposited (type <= 2)                                */
+  long n_dead;              /* galaxies skipped (type > 2)                                   */
+  long n_bh_below_limit;    /* N_BlackHoleMassLimitReion (bh_feedback, :1473-1478)           */
+  double sfr_timescale;     /* the divisor used                                              */
+} mrx_grid_stats;
+
+/* Replaces construct_baryon_grids(snapshot, local_ngals) (reionization.c:1353),
+ * with the galaxy fields as arrays (galaxy_t, meraxes.h:547-635: Pos, Type,
+ * FescWeightedGSM, GrossStellarMass, BlackHoleMass, EffectiveBHM) in the
+ * order of galaxy_to_slab_map.  pos is [n_gal][3] floats in Mpc/h;
+ * gross_stellar_mass is needed with include_spin_temp, black_hole_mass and
+ * effective_bhm with bh_feedback (otherwise nullable); sfr and sfr_histories
+ * with include_spin_temp.  n_gal < 2^31. */
+int mrx_construct_baryon_grids(mrx_ctx* ctx, const mrx_grid_params* p, long n_gal, const float* pos,
+                               const int* type, const double* fesc_weighted_gsm, const double* gross_stellar_mass,
+                               const double* black_hole_mass, const double* effective_bhm, float* stars,
+                               float* weighted_sfr, float* sfr, float* sfr_histories, mrx_grid_stats* stats);
+
 /* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params,
- * 3 mrx_ts_params, 4 mrx_ts_globals, 5 mrx_hii_batch_box),
- * so a binding can check that its mirror of the layout matches this build. */
+ * 3 mrx_ts_params, 4 mrx_ts_globals, 5 mrx_hii_batch_box, 6 mrx_grid_params,
+ * 7 mrx_grid_stats), so a binding can check that its mirror of the layout
+ * matches this build. */
 size_t mrx_abi_sizeof(int which);
 
 #ifdef __cplusplus

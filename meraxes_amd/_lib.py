@@ -6,7 +6,7 @@ entry point raises.  Build it with ``python -m meraxes_amd.build``.
 import ctypes as C
 import os
 
-from .abi import ERRORS, HiiBatchBox, HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams
+from .abi import ERRORS, GridParams, GridStats, HiiBatchBox, HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmrx_hip.so")
@@ -17,7 +17,7 @@ EXPORTS = (
     "mrx_find_hii_bubbles", "mrx_filter", "mrx_fft_r2c_3d", "mrx_fft_c2r_3d", "mrx_filtered_field",
     "mrx_brightness_temp", "mrx_abi_sizeof", "mrx_profile", "mrx_profile_read", "mrx_compute_ts",
     "mrx_hii_batch_create", "mrx_hii_batch_run", "mrx_hii_batch_describe", "mrx_hii_batch_free",
-    "mrx_power_spectrum", "mrx_power_spectrum_adjoint", "mrx_snapshot_adjoint",
+    "mrx_power_spectrum", "mrx_power_spectrum_adjoint", "mrx_snapshot_adjoint", "mrx_construct_baryon_grids",
 )
 
 _lib = None
@@ -62,6 +62,8 @@ def load():
     L.mrx_power_spectrum.argtypes = [ctxp, i, d, f, vp, i, vp, vp, vp, vp]
     L.mrx_power_spectrum_adjoint.argtypes = [ctxp, i, d, f, vp, vp, vp]
     L.mrx_snapshot_adjoint.argtypes = [ctxp, vp, C.POINTER(HiiParams), C.POINTER(TbParams)] + [vp] * 16
+    L.mrx_construct_baryon_grids.argtypes = ([ctxp, C.POINTER(GridParams), C.c_long] + [vp] * 10 +
+                                             [C.POINTER(GridStats)])
     L.mrx_abi_sizeof.argtypes = [i]
     L.mrx_abi_sizeof.restype = C.c_size_t
     _lib = L

@@ -166,3 +166,37 @@ class HiiBatchBox(C.Structure):
         ("z_at_ionization", C.c_void_p),
         ("J_21_at_ionization", C.c_void_p),
     ]
+
+
+class GridParams(C.Structure):
+    """mrx_grid_params: knobs of construct_baryon_grids (reionization.c:1353-1615)."""
+
+    _fields_ = [
+        ("grid_dim", C.c_int),
+        ("include_spin_temp", C.c_int),
+        ("bh_feedback", C.c_int),
+        ("n_store_sfr", C.c_int),
+        ("pointers_on_device", C.c_int),
+        ("pad_", C.c_int),
+        ("box_size", C.c_double),
+        ("reion_sfr_timescale", C.c_double),
+        ("hubble", C.c_double),
+        ("omega_m", C.c_double),
+        ("omega_k", C.c_double),
+        ("omega_lambda", C.c_double),
+        ("redshift", C.c_double),
+        ("bh_mass_limit_reion", C.c_double),
+        ("alpha_uv_bh", C.c_double),
+        ("alpha_uv", C.c_double),
+    ]
+
+
+class GridStats(C.Structure):
+    """mrx_grid_stats: galaxy counts of one gridding call and the SFR timescale used."""
+
+    _fields_ = [
+        ("n_gridded", C.c_long),
+        ("n_dead", C.c_long),
+        ("n_bh_below_limit", C.c_long),
+        ("sfr_timescale", C.c_double),
+    ]

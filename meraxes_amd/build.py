@@ -45,7 +45,8 @@ def build(verbose=False):
         raise RuntimeError(f"gcc failed:\n{r.stderr}")
     # the HIP translation unit and the C object are linked together through
     # the linker so hipcc does not treat the object as a HIP source
-    cmd = [hipcc()] + flags() + ["-o", tmp, src, "-Xlinker", obj, "-lpthread"]
+    grid = os.path.join(CSRC, "grid.hip")  # galaxy gridding (rocPRIM sort): its own translation unit
+    cmd = [hipcc()] + flags() + ["-o", tmp, src, grid, "-Xlinker", obj, "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

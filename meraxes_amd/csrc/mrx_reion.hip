@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/mrx_reion.h"
+#include "ctx.hpp"
 #include "reion_kernels.hpp"
 #include "ts_kernels.hpp"
 #include "adjoint.hpp"
@@ -82,6 +83,9 @@ struct mrx_ctx {
   size_t adj_work_cap = 0;
   double* adj_gS = nullptr;
   size_t adj_gS_cap = 0;
+  // buffers of the other translation units (ctx_buffer)
+  void* ext[mrx::kCtxSlots] = {};
+  size_t ext_cap[mrx::kCtxSlots] = {};
 };
 
 namespace {
@@ -1163,6 +1167,53 @@ int capture_batch(mrx_ctx* ctx, mrx_hii_batch* b)
 }  // namespace
 
 // =============================================================== C ABI
+// ------------------------------------------- context access for other units
+namespace mrx {
+hipStream_t ctx_stream(mrx_ctx* c) { return c->stream; }
+int ctx_device(mrx_ctx* c) { return c->device; }
+int ctx_fail(mrx_ctx* c, int code, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  c->err = buf;
+  return code;
+}
+void* ctx_buffer(mrx_ctx* c, int slot, size_t bytes)
+{
+  if (slot < 0 || slot >= kCtxSlots) {
+    ctx_fail(c, MRX_EINVAL, "ctx_buffer: slot %d out of range", slot);
+    return nullptr;
+  }
+  if (c->ext[slot] && c->ext_cap[slot] >= bytes) return c->ext[slot];
+  if (c->ext[slot]) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->ext[slot]);
+    c->ext[slot] = nullptr;
+    c->ext_cap[slot] = 0;
+  }
+  const size_t nb = bytes ? bytes : 1;
+  if (hipMalloc(&c->ext[slot], nb) != hipSuccess) {
+    (void)hipGetLastError();
+    c->ext[slot] = nullptr;
+    ctx_fail(c, MRX_ENOMEM, "device allocation of %zu bytes failed", nb);
+    return nullptr;
+  }
+  c->ext_cap[slot] = nb;
+  return c->ext[slot];
+}
+int ctx_finish(mrx_ctx* c)
+{
+  if (!c->user_stream) {
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return ctx_fail(c, MRX_EDEVICE, "stream synchronisation failed: %s", hipGetErrorString(e));
+  }
+  return MRX_OK;
+}
+}  // namespace mrx
+
 extern "C" {
 
 int mrx_init(int device, mrx_ctx** out)
@@ -1204,6 +1255,7 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->wtab);
   dfree(ctx->adj_work);
   dfree(ctx->adj_gS);
+  for (auto& e : ctx->ext) dfree(e);
   delete ctx->tsd;
   delete ctx->tss;
   delete ctx->ts_host;
@@ -1769,6 +1821,8 @@ size_t mrx_abi_sizeof(int which)
     case 3: return sizeof(mrx_ts_params);
     case 4: return sizeof(mrx_ts_globals);
     case 5: return sizeof(mrx_hii_batch_box);
+    case 6: return sizeof(mrx_grid_params);
+    case 7: return sizeof(mrx_grid_stats);
     default: return 0;
   }
 }

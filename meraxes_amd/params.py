@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .abi import HiiParams, TbParams, TsParams
+from .abi import GridParams, HiiParams, TbParams, TsParams
 
 # src/meraxes.h:21-44
 GRAVITY = 6.672e-8
@@ -68,6 +68,29 @@ def hii_params(grid_dim, box_size, redshift, filter_type=0, efficiency=20.0, uvb
     p.unit_length_cm = UNIT_LENGTH_CM
     p.unit_mass_g = UNIT_MASS_G
     p.unit_time_s = u["unit_time_s"]
+    return p
+
+
+def grid_params(grid_dim, box_size, redshift, include_spin_temp=0, n_store_sfr=8, bh_feedback=1,
+                reion_sfr_timescale=0.5, bh_mass_limit_reion=-1.0, alpha_uv_bh=1.57, alpha_uv=5.0, on_device=0):
+    """mrx_grid_params (defaults.par:183-191, :227-228, :248 for the knobs; Tiamat.par cosmology)."""
+    u = units()
+    p = GridParams()
+    p.grid_dim = grid_dim
+    p.include_spin_temp = include_spin_temp
+    p.bh_feedback = bh_feedback
+    p.n_store_sfr = n_store_sfr
+    p.pointers_on_device = on_device
+    p.box_size = box_size
+    p.reion_sfr_timescale = reion_sfr_timescale
+    p.hubble = HUBBLE * u["unit_time_s"]  # run_globals.Hubble (init.c:201)
+    p.omega_m = OMEGA_M
+    p.omega_k = 0.0
+    p.omega_lambda = OMEGA_L
+    p.redshift = redshift
+    p.bh_mass_limit_reion = bh_mass_limit_reion
+    p.alpha_uv_bh = alpha_uv_bh
+    p.alpha_uv = alpha_uv
     return p
 
 

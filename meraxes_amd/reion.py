@@ -26,7 +26,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import Context, MrxError, load  # noqa: F401
-from .abi import HiiBatchBox, HiiGlobals, TsGlobals, TsParams
+from .abi import GridStats, HiiBatchBox, HiiGlobals, TsGlobals, TsParams
 
 try:  # torch is optional for host-memory use
     import torch
@@ -231,6 +231,54 @@ def compute_ts(ctx, params, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_
     out = {k: getattr(g, k) for k, _ in TsGlobals._fields_}
     out.update({"x_e_box": x_e_box, "x_e_box_prev": x_e_box_prev, "Tk_box": Tk_box, "TS_box": TS_box})
     return out
+
+
+def construct_baryon_grids(ctx, params, pos, type_, fesc_weighted_gsm, gross_stellar_mass=None, black_hole_mass=None,
+                           effective_bhm=None, stars=None, weighted_sfr=None, sfr=None, sfr_histories=None):
+    """Galaxies to the padded stars / weighted_sfr (/ sfr) grids (reionization.c:1353-1615).
+
+    The galaxy fields are arrays in galaxy_to_slab_map order: ``pos`` [n, 3]
+    float32 in Mpc/h, ``type_`` int32, the masses float64 - numpy arrays (host)
+    or CUDA tensors (device), all alike.  With params.include_spin_temp,
+    ``sfr_histories`` ([n_store_sfr] padded grids) is updated in place: moved by
+    one slot, slot 0 = the new sfr grid.  Returns the grids and the counts.
+    """
+    N = params.grid_dim
+    on_dev = _is_dev(pos)
+    params.pointers_on_device = 1 if on_dev else 0
+    n = int(pos.shape[0]) if pos is not None else 0
+    want = {"pos": (pos, "float32"), "type": (type_, "int32"), "fesc_weighted_gsm": (fesc_weighted_gsm, "float64"),
+            "gross_stellar_mass": (gross_stellar_mass, "float64"), "black_hole_mass": (black_hole_mass, "float64"),
+            "effective_bhm": (effective_bhm, "float64")}
+    for k, (a, dt) in want.items():
+        if a is None:
+            continue
+        if _is_dev(a) != on_dev:
+            raise ValueError(f"{k}: galaxy arrays must all be host or all be device memory")
+        if str(a.dtype).replace("torch.", "") != dt:
+            raise ValueError(f"{k} must be {dt}, got {a.dtype}")
+        if (a.shape[0] if a.ndim else 0) != n or (k == "pos" and tuple(a.shape[1:]) != (3,)):
+            raise ValueError(f"{k} has shape {tuple(a.shape)} for {n} galaxies")
+    shape = (N, N, 2 * (N // 2 + 1))
+
+    def grid():
+        if on_dev:
+            return torch.empty(shape, dtype=torch.float32, device=pos.device)
+        return np.empty(shape, np.float32)
+
+    stars = grid() if stars is None else stars
+    weighted_sfr = grid() if weighted_sfr is None else weighted_sfr
+    if params.include_spin_temp and sfr is None:
+        sfr = grid()
+    st = GridStats()
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_construct_baryon_grids(
+        ctx.ptr, C.byref(params), n, _ptr(pos), _ptr(type_), _ptr(fesc_weighted_gsm), _ptr(gross_stellar_mass),
+        _ptr(black_hole_mass), _ptr(effective_bhm), _ptr(stars), _ptr(weighted_sfr), _ptr(sfr), _ptr(sfr_histories),
+        C.byref(st)), "construct_baryon_grids")
+    return {"stars": stars, "weighted_sfr": weighted_sfr, "sfr": sfr, "sfr_histories": sfr_histories,
+            "n_gridded": st.n_gridded, "n_dead": st.n_dead, "n_bh_below_limit": st.n_bh_below_limit,
+            "sfr_timescale": st.sfr_timescale}
 
 
 def power_spectrum(ctx, delta_T, box_size_mpc, k_factor=1.35, max_bins=64):

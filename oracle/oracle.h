@@ -81,6 +81,13 @@ int orc_brightness_temp(const mrx_tb_params* p, const float* xH, const float* de
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
+/* construct_baryon_grids restated (reionization.c:1353-1615), one rank, host
+ * arrays; stats = {n_gridded, n_dead, n_bh_below_limit}.  Returns 0, or
+ * -100 - g for the first galaxy g (capped at 1e6) with a negative grid index. */
+int orc_construct_baryon_grids(const mrx_grid_params* p, long n_gal, const float* pos, const int* type,
+                               const double* fesc, const double* gross, const double* bhm, const double* ebhm,
+                               float* stars, float* wsfr, float* sfr, float* hist, long* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,7 @@ import subprocess
 
 import numpy as np
 
-from meraxes_amd.abi import HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams, fptr
+from meraxes_amd.abi import GridParams, HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams, fptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -47,6 +47,7 @@ def lib():
         L.orc_brightness_temp.argtypes = [C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
         L.orc_compute_ts.argtypes = [C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals), vp]
         L.orc_ts_export.argtypes = [C.POINTER(TsParams), vp, i]
+        L.orc_construct_baryon_grids.argtypes = [C.POINTER(GridParams), C.c_long] + [vp] * 11
         L.orc_set_threads.argtypes = [i]
         L.orc_get_threads.restype = i
         _lib = L
@@ -228,3 +229,31 @@ def ts_export(p):
         k += 4
     out["ops"] = ops
     return out
+
+
+def construct_baryon_grids(p, pos, type_, fesc, gross=None, bhm=None, ebhm=None, hist=None):
+    """construct_baryon_grids restated (reionization.c:1353-1615) on host arrays.
+
+    hist (spin temperature: [n_store_sfr] padded grids) is copied, shifted and
+    returned.  Returns stars, weighted_sfr, sfr, hist and the counts.
+    """
+    N = p.grid_dim
+    shape = (N, N, 2 * (N // 2 + 1))
+    pos = np.ascontiguousarray(pos, np.float32)
+    type_ = np.ascontiguousarray(type_, np.int32)
+    fesc = np.ascontiguousarray(fesc, np.float64)
+    conv = (lambda a: None if a is None else np.ascontiguousarray(a, np.float64))
+    gross, bhm, ebhm = conv(gross), conv(bhm), conv(ebhm)
+    stars = np.empty(shape, np.float32)
+    wsfr = np.empty(shape, np.float32)
+    sfr = np.empty(shape, np.float32) if p.include_spin_temp else None
+    hist = None if hist is None else np.array(hist, np.float32, copy=True)
+    stats = np.zeros(3, np.int64)
+    rc = lib().orc_construct_baryon_grids(C.byref(p), len(pos), fptr(pos), fptr(type_), fptr(fesc), fptr(gross),
+                                          fptr(bhm), fptr(ebhm), fptr(stars), fptr(wsfr), fptr(sfr), fptr(hist),
+                                          fptr(stats))
+    if rc <= -100:
+        raise ValueError(f"galaxy {-100 - rc} has a negative grid index")
+    _chk(rc, "construct_baryon_grids")
+    return {"stars": stars, "weighted_sfr": wsfr, "sfr": sfr, "sfr_histories": hist, "n_gridded": int(stats[0]),
+            "n_dead": int(stats[1]), "n_bh_below_limit": int(stats[2])}

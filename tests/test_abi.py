@@ -30,6 +30,27 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_lib.EXPORTS)
 
 
+def declared_arity():
+    """{function: number of parameters} from include/mrx_reion.h."""
+    src = open(os.path.join(ROOT, "include", "mrx_reion.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(mrx_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", src, flags=re.S):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_bindings_match_declared_arity():
+    L = _lib.load()
+    ar = declared_arity()
+    assert set(ar) == set(_lib.EXPORTS)
+    for name, n in ar.items():
+        at = getattr(L, name).argtypes
+        assert at is not None, name
+        assert len(at) == n, (name, len(at), n)
+
+
 def test_struct_mirrors_match_library():
     L = _lib.load()
     assert L.mrx_abi_sizeof(0) == C.sizeof(abi.HiiParams)
@@ -38,6 +59,8 @@ def test_struct_mirrors_match_library():
     assert L.mrx_abi_sizeof(3) == C.sizeof(abi.TsParams)
     assert L.mrx_abi_sizeof(4) == C.sizeof(abi.TsGlobals)
     assert L.mrx_abi_sizeof(5) == C.sizeof(abi.HiiBatchBox)
+    assert L.mrx_abi_sizeof(6) == C.sizeof(abi.GridParams)
+    assert L.mrx_abi_sizeof(7) == C.sizeof(abi.GridStats)
 
 
 def test_radius_ladder_matches_oracle_and_closed_form():
