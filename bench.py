@@ -19,7 +19,8 @@ oracle (the C restatement of the reference's FFTW-float path) on the host cores
 over a bounded sample of the same workload.  Unless --no-extras is given the
 snapshot run also times the hand-written adjoint against its forward at 256^3
 and 512^3, the batched bubble finder (96 uneven boxes, one graph launch)
-against the per-box loop, and the galaxy gridding (2^24 galaxies onto 512^3).
+against the per-box loop, the galaxy gridding (2^24 galaxies onto 512^3) and
+delta_T with peculiar velocities and line-of-sight RSD at 512^3.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512]
            [--workload snapshot|bubbles|batched] [--no-extras] [--no-cpu-baseline]
@@ -292,6 +293,56 @@ def bench_gridding(ctx, N=512, n_gal=1 << 24, reps=3):
             "grid_GB_s": grid_bytes / dt / 1e9,
             "config": f"{n_gal} galaxies, {N}^3, {L:g} Mpc/h, spin temperature and black-hole feedback on, "
                       f"a tenth in 16 clumps, synthetic (torch generator)"}
+
+
+def bench_tb_vel(ctx, N=512, reps=3):
+    """delta_T with peculiar velocities (Flag_IncludePecVelsFor21cm = 3) at 512^3.
+
+    Velocity conversion, FFT gradient, optical-depth delta_T and the
+    line-of-sight sub-cell re-binning (50 sub-cells per cell); a smooth random
+    velocity field of 0.25 cell rms displacement, grids resident in HBM.
+    """
+    import math
+
+    import torch
+
+    from meraxes_amd import params as P
+    from meraxes_amd import reion
+
+    L = 400.0 * N / 512
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cuda").manual_seed(21)
+    xH = torch.rand((N, N, N), generator=g, device=dev)
+    deltax = torch.randn((N, N, N + 2), generator=g, device=dev) * 0.3
+    TS = torch.rand((N, N, N), generator=g, device=dev) * 200 + 15
+    z = 8.0
+    Hz = 3.2407789e-18 * P.HUBBLE_H * math.sqrt(P.OMEGA_M * (1 + z) ** 3 + P.OMEGA_L)
+    unit = (1 + z) * 1e5 / 3.086e24 * (1 + z)
+    # smooth field: box-filtered noise along z, displacements of 0.25 cell rms
+    # (a 30 km/s peculiar velocity at z = 8 moves gas by about 0.1-0.3 cells here)
+    v0 = torch.randn((N, N, N + 2), generator=g, device=dev)
+    v0 = (v0 + torch.roll(v0, 1, 2) + torch.roll(v0, -1, 2) + torch.roll(v0, 2, 2)) / 8.0
+    v0 *= float((L / P.HUBBLE_H) / N * Hz / unit)
+    vel = torch.empty_like(v0)
+    p = P.tb_vel_params(N, L, z, pec_vels=3, trees_id=0)
+    out = torch.empty((N, N, N), device=dev)
+    vel.copy_(v0)
+    reion.brightness_temp_vel(ctx, p, xH, deltax, TS, vel, out=out)
+    torch.cuda.synchronize()
+    t = 0.0
+    for _ in range(reps):
+        vel.copy_(v0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reion.brightness_temp_vel(ctx, p, xH, deltax, TS, vel, out=out)
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+    dt = t / reps
+    log(f"[bench] delta_T with velocities (RSD) at {N}^3: {dt * 1e3:.2f} ms")
+    del xH, deltax, TS, v0, vel, out
+    torch.cuda.empty_cache()
+    return {"ms_per_call": dt * 1e3, "cells_per_s": N**3 / dt, "grid": N, "pec_vels": 3,
+            "config": f"{N}^3, {L:g} Mpc/h, z=8, spin temperature, RSD with 50 sub-cells, synthetic"}
 
 
 BATCH_PER_SIZE = 32
@@ -588,6 +639,7 @@ def main():
         out["adjoint"] = {str(n): bench_adjoint(ctx, n) for n in (256, N)}
         out["batched"] = bench_batched(ctx, 2, 1)
         out["gridding"] = bench_gridding(ctx)
+        out["tb_velocities"] = bench_tb_vel(ctx)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(snapshot)

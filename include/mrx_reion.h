@@ -330,6 +330,35 @@ int mrx_hii_batch_describe(const mrx_hii_batch* batch, char* buf, int len);
 int mrx_hii_batch_free(mrx_hii_batch* batch);
 
 /*
+ * Peculiar-velocity branch of ComputeBrightnessTemperatureBox
+ * (BrightnessTemperature.c:126-510), applied (as in the reference) only with
+ * include_spin_temp and pec_vels > 0; otherwise the call is mrx_brightness_temp.
+ * vel (padded) is converted in place to comoving velocity (:166-195, by
+ * trees_id); its z-gradient is formed by FFT (/N^3, *Hubble_h, *i k_z,
+ * velocity_gradient reionization.c:2172-2187); then
+ *   pec_vels 1: delta_T /= (dv/dz capped at |MAX_DVDR H(z)|) / H(z) + 1        (:210-236)
+ *   pec_vels 2: optical depth tau = delta_T (1+z)/(1000 T_S) / |dv/dz / H(z) + 1|,
+ *               delta_T = (1 - e^-tau) 1000 (T_S - T_CMB(1+z)) / (1+z)       (:240-287)
+ *   pec_vels 3: 2, then line-of-sight redshift-space distortions: every cell
+ *               split into N_RSD_STEPS = 50 sub-cells moved by the interpolated
+ *               displacement v / H(z) and re-binned along z (:312-496).
+ */
+typedef struct mrx_tb_vel_params {
+  mrx_tb_params tb;      /* as for mrx_brightness_temp                                         */
+  int pec_vels;          /* Flag_IncludePecVelsFor21cm: 0..3                                   */
+  int trees_id;          /* TreesID: 0 VELOCIRAPTOR, 1 GBPTREES, 2 VELOCIRAPTOR_AUG (:171-182)  */
+  double box_size;       /* BoxSize [Mpc/h]                                                    */
+  double omega_lambda;   /* OmegaLambda  } hubble(z), XRayHeatingFunctions.c:394-404           */
+  double omega_r;        /* OmegaR       }                                                     */
+} mrx_tb_vel_params;
+
+/* Replaces ComputeBrightnessTemperatureBox (BrightnessTemperature.c:24) with
+ * Flag_IncludePecVelsFor21cm > 0.  vel: padded velocity grid, converted in
+ * place as the reference does. */
+int mrx_brightness_temp_vel(mrx_ctx* ctx, const mrx_tb_vel_params* p, const float* xH, const float* deltax,
+                            const float* TS_box, float* vel, float* delta_T, double* ave_Tb);
+
+/*
  * Galaxy gridding: construct_baryon_grids (reionization.c:1353-1615) for one
  * process holding the whole box.  Each galaxy that is not dead (type <= 2) is
  * deposited on its nearest grid point, pos_to_ngp (misc_tools.c:163-173):
@@ -389,8 +418,8 @@ int mrx_construct_baryon_grids(mrx_ctx* ctx, const mrx_grid_params* p, long n_ga
 
 /* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params,
  * 3 mrx_ts_params, 4 mrx_ts_globals, 5 mrx_hii_batch_box, 6 mrx_grid_params,
- * 7 mrx_grid_stats), so a binding can check that its mirror of the layout
- * matches this build. */
+ * 7 mrx_grid_stats, 8 mrx_tb_vel_params), so a binding can check that its
+ * mirror of the layout matches this build. */
 size_t mrx_abi_sizeof(int which);
 
 #ifdef __cplusplus

@@ -200,3 +200,16 @@ class GridStats(C.Structure):
         ("n_bh_below_limit", C.c_long),
         ("sfr_timescale", C.c_double),
     ]
+
+
+class TbVelParams(C.Structure):
+    """mrx_tb_vel_params: ComputeBrightnessTemperatureBox with peculiar velocities."""
+
+    _fields_ = [
+        ("tb", TbParams),
+        ("pec_vels", C.c_int),
+        ("trees_id", C.c_int),
+        ("box_size", C.c_double),
+        ("omega_lambda", C.c_double),
+        ("omega_r", C.c_double),
+    ]

@@ -29,6 +29,7 @@ using namespace mrx;
 namespace {
 // constants of src/meraxes.h:21-44 and reionization.h:38
 constexpr double kPlanck = 6.6262e-27;
+constexpr double kHubble = 3.2407789e-18;
 constexpr double kProtonMass = 1.6726e-24;
 constexpr double kTcmb = 2.728;
 constexpr double kLFactor = 0.620350491;
@@ -83,6 +84,9 @@ struct mrx_ctx {
   size_t adj_work_cap = 0;
   double* adj_gS = nullptr;
   size_t adj_gS_cap = 0;
+  // brightness temperature with velocities: gradient grid + line-of-sight bins
+  float* tbv = nullptr;
+  size_t tbv_cap = 0;
   // buffers of the other translation units (ctx_buffer)
   void* ext[mrx::kCtxSlots] = {};
   size_t ext_cap[mrx::kCtxSlots] = {};
@@ -927,6 +931,88 @@ __global__ void k_sum_f32(const float* __restrict__ x, long n, double* __restric
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
+struct TbVel {
+  template <int N>
+  struct At {
+    // delta_T with peculiar velocities; device grids; vel converted in place
+    static int run(mrx_ctx* ctx, const mrx_tb_vel_params* p, const float* xH, const float* deltax, const float* TS,
+                   float* vel, float* dT, double* ave)
+    {
+      using P = Pipe<N>;
+      constexpr int NH = N / 2;
+      const long nr = (long)N * N * N;
+      const size_t np = npad(N);
+      MRX_TRY(ensure_buffers(ctx, N, 1));
+      MRX_TRY(ensure_buf(ctx, ctx->tbv, ctx->tbv_cap, np + (p->pec_vels == 3 ? (size_t)nr : 0)));
+      float* grad = ctx->tbv;
+      const TbArgs base = tb_args(&p->tb);
+      TbVelArgs a;
+      a.const_factor = base.const_factor;
+      a.redshift = (float)(float)p->tb.redshift;
+      a.N = N;
+      a.pec_vels = p->pec_vels;
+      a.trees_id = p->trees_id;
+      a.T_rad = base.T_rad;
+      // hubble(z), XRayHeatingFunctions.c:394-404 (z a float there)
+      const float z = a.redshift;
+      a.H_z = (float)(float)(kHubble * p->tb.hubble_h *
+                             sqrt(p->tb.omega_m * std::pow((double)(1 + z), 3.0) +
+                                  p->omega_r * std::pow((double)(1 + z), 4.0) + p->omega_lambda));
+      a.max_v_deriv = fabs(0.2f * a.H_z);
+      const unsigned nb = (unsigned)((nr + kNT - 1) / kNT);
+      {
+        PROF("vel_convert");
+        k_vel_convert<<<nb, kNT, 0, ctx->stream>>>(vel, N, a.redshift, a.trees_id);
+      }
+      MRX_TRY(check_launch(ctx, "k_vel_convert"));
+      MRX_TRY(P::forward(ctx, vel, 0, 1, true, ctx->U, ctx->V));
+      const float box_f = (float)p->box_size;
+      const float delta_k = (float)(2.0 * M_PI / box_f);
+      {
+        PROF("vel_grad_op");
+        k_vel_grad_op<<<(unsigned)((P::nc + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->U, ctx->T, N,
+                                                                                  p->tb.hubble_h, delta_k);
+      }
+      MRX_TRY(check_launch(ctx, "k_vel_grad_op"));
+      MRX_TRY(P::inv_xy_plain(ctx, ctx->T, 1));
+      MRX_TRY(P::c2r_z(ctx, ctx->T, grad, 1, -1, N + 2, NH));
+      {
+        PROF("brightness_temp_vel");
+        k_brightness_temp_vel<<<nb, kNT, 0, ctx->stream>>>(xH, deltax, TS, grad, dT, a);
+      }
+      MRX_TRY(check_launch(ctx, "k_brightness_temp_vel"));
+      if (p->pec_vels == 3) {
+        RsdArgs r;
+        r.N = N;
+        r.H_z = a.H_z;
+        const double box_mpc = p->box_size / p->tb.hubble_h;  // BrightnessTemperature.c:44
+        const double sub_w = (box_mpc / ((double)N)) / (double)kRsdSteps;
+        r.cell = box_mpc / (float)N;
+        r.half = (sub_w / 2.) / r.cell;
+        r.frac_w = sub_w / r.cell;
+        for (int s = 0; s < kRsdSteps; ++s) {
+          r.x_off[s] = (float)(sub_w * (float)s + sub_w / 2.);
+          r.x_pos[s] = (float)(r.x_off[s] / (box_mpc / (float)N));
+        }
+        PROF("rsd_los");
+        k_rsd_los<<<(unsigned)(((long)N * N + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(dT, xH, vel, grad + np, r);
+        MRX_TRY(check_launch(ctx, "k_rsd_los"));
+      }
+      MRX_TRY(ensure_partials(ctx, 1024 + 8));
+      if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
+      k_sum_f32<<<1024, kNT, 0, ctx->stream>>>(dT, nr, ctx->partials);
+      MRX_TRY(check_launch(ctx, "k_sum_f32"));
+      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, 1024, 1, ctx->sums);
+      MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+      double h;
+      HIP_TRY(hipMemcpyAsync(&h, ctx->sums, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      if (ave) *ave = h / pow((double)N, 3);
+      return MRX_OK;
+    }
+  };
+};
+
 struct PowerSpec {
   template <int N>
   struct At {
@@ -1255,6 +1341,7 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->wtab);
   dfree(ctx->adj_work);
   dfree(ctx->adj_gS);
+  dfree(ctx->tbv);
   for (auto& e : ctx->ext) dfree(e);
   delete ctx->tsd;
   delete ctx->tss;
@@ -1558,6 +1645,47 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
   return MRX_OK;
 }
 
+int mrx_brightness_temp_vel(mrx_ctx* ctx, const mrx_tb_vel_params* p, const float* xH, const float* deltax,
+                            const float* TS_box, float* vel, float* delta_T, double* ave_Tb)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!p) return set_err(ctx, MRX_EINVAL, "brightness_temp_vel: null params");
+  // velocities act only with the spin temperature (BrightnessTemperature.c:159)
+  if (!p->tb.include_spin_temp || p->pec_vels <= 0)
+    return mrx_brightness_temp(ctx, &p->tb, xH, deltax, TS_box, delta_T, ave_Tb);
+  const int N = p->tb.grid_dim;
+  if (p->pec_vels > 3) return set_err(ctx, MRX_EINVAL, "pec_vels %d: Flag_IncludePecVelsFor21cm is 0..3", p->pec_vels);
+  if (!xH || !deltax || !TS_box || !vel || !delta_T) return set_err(ctx, MRX_EINVAL, "brightness_temp_vel: null grid");
+  if (!supported_n(N)) return set_err(ctx, MRX_EUNSUPPORTED, "grid_dim %d not built", N);
+  if (!(p->box_size > 0)) return set_err(ctx, MRX_EINVAL, "brightness_temp_vel: box_size must be > 0");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t np = npad(N), nr = nreal(N);
+  const float *dxH = xH, *ddx = deltax, *dTS = TS_box;
+  float *dvel = vel, *ddT = delta_T;
+  if (!p->tb.pointers_on_device) {
+    MRX_TRY(ensure_stage(ctx, 2 * np + 3 * nr));
+    float* s = ctx->stage;
+    HIP_TRY(hipMemcpyAsync(s, deltax, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s + np, vel, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s + 2 * np, xH, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s + 2 * np + nr, TS_box, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+    ddx = s;
+    dvel = s + np;
+    dxH = s + 2 * np;
+    dTS = s + 2 * np + nr;
+    ddT = s + 2 * np + 2 * nr;
+  }
+  double ave = 0.0;
+  MRX_TRY(dispatch<TbVel>(N, ctx, p, dxH, ddx, dTS, dvel, ddT, &ave));
+  if (!p->tb.pointers_on_device) {
+    HIP_TRY(hipMemcpyAsync(vel, dvel, sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(delta_T, ddT, sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (ave_Tb) *ave_Tb = ave;
+  return finish(ctx);
+}
+
 int mrx_profile(mrx_ctx* ctx, int enable)
 {
   if (!ctx) return MRX_EINVAL;
@@ -1823,6 +1951,7 @@ size_t mrx_abi_sizeof(int which)
     case 5: return sizeof(mrx_hii_batch_box);
     case 6: return sizeof(mrx_grid_params);
     case 7: return sizeof(mrx_grid_stats);
+    case 8: return sizeof(mrx_tb_vel_params);
     default: return 0;
   }
 }

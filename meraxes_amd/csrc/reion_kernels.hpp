@@ -939,4 +939,197 @@ __global__ __launch_bounds__(kNT) void k_brightness_temp(const float* __restrict
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
+// ----------------------------------- brightness temperature with velocities
+// Peculiar-velocity branch of ComputeBrightnessTemperatureBox
+// (BrightnessTemperature.c:126-510), every operation in the reference's
+// precision and order (fp contract off).
+#pragma clang fp contract(off)
+constexpr int kRsdSteps = 50;  // N_RSD_STEPS (reionization.h:34)
+
+struct TbVelArgs {
+  float const_factor;  // :50-51
+  float redshift;      // (float)ZZ[snapshot]
+  float H_z;           // (float)hubble(redshift)
+  int N;
+  int pec_vels;
+  int trees_id;
+  double T_rad;
+  double max_v_deriv;  // fabs(MAX_DVDR * H_z)
+};
+
+// vel (padded, real cells) to comoving velocity in place (:166-195)
+__global__ __launch_bounds__(kNT) void k_vel_convert(float* __restrict__ vel, int N, float redshift, int trees_id)
+{
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= (long)N * N * N) return;
+  float* v = vel + (i / N) * (N + 2) + i % N;
+  float x = *v;
+  if (trees_id == 1)
+    x = (float)(sqrt(1. / (1. + redshift)) * (double)x / 1000.);
+  else if (trees_id == 0 || trees_id == 2)
+    x = (float)((1. + redshift) * (double)x);
+  x = (float)((double)x * (1000. * 100. / 3.086e24));
+  x = (float)((double)x / (1. / (1. + redshift)));
+  *v = x;
+}
+
+// Packed spectrum U (forward transform / N^3) -> T = U * Hubble_h * i k_z
+// (:199-205, velocity_gradient reionization.c:2172-2187).  The kz = 0 plane is
+// multiplied by 0; the Nyquist plane's product i k B is purely imaginary after
+// the inverse x/y passes (B is Hermitian), and the c2r keeps only the real
+// part of the Nyquist coefficient, so it contributes 0: the packed kz = 0 slot
+// of T is 0.
+__global__ __launch_bounds__(kNT) void k_vel_grad_op(const float2* __restrict__ U, float2* __restrict__ T, int N,
+                                                    double hubble_h, float delta_k)
+{
+  const int NH = N / 2;
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= (long)N * N * NH) return;
+  const int kz = (int)(i % NH);
+  if (kz == 0) {
+    T[i] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  const float2 c = U[i];
+  const float re = (float)((double)c.x * hubble_h), im = (float)((double)c.y * hubble_h);
+  const float k_z = (float)kz * delta_k;
+  T[i] = make_float2(re * 0.0f - im * k_z, re * k_z + im * 0.0f);
+}
+
+// delta_T with T_S and the velocity gradient grad (padded), pec_vels 1 or >= 2 (:86-120, :210-287)
+__global__ __launch_bounds__(kNT) void k_brightness_temp_vel(const float* __restrict__ xH,
+                                                            const float* __restrict__ deltax,
+                                                            const float* __restrict__ TS,
+                                                            const float* __restrict__ grad, float* __restrict__ dT,
+                                                            TbVelArgs a)
+{
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= (long)a.N * a.N * a.N) return;
+  const long ip = (i / a.N) * (a.N + 2) + i % a.N;
+  const double pixel_deltax = deltax[ip], pixel_x_HI = xH[i];
+  float v = (float)(a.const_factor * pixel_x_HI * (1.0 + pixel_deltax));
+  const float ts = TS[i];
+  if (a.pec_vels > 1)
+    v = (float)((double)v * ((1. + a.redshift) / (1000. * (double)ts)));
+  else
+    v = (float)((double)v * (1 - a.T_rad / (double)ts));
+  if (a.pec_vels == 1) {
+    double dvdx = (double)grad[ip];
+    if (fabs(dvdx) > a.max_v_deriv) dvdx = (dvdx < 0) ? -a.max_v_deriv : a.max_v_deriv;
+    v = (float)((double)v / (dvdx / (double)a.H_z + 1.0));
+  } else {
+    const double gc = fabs((double)grad[ip] / (double)a.H_z + 1.0);
+    if (gc < 1e-7)
+      v = (float)(1000. * ((double)ts - a.T_rad) / (1. + a.redshift));
+    else
+      v = (float)((1. - exp(-(double)v / gc)) * 1000. * ((double)ts - a.T_rad) / (1. + a.redshift));
+  }
+  dT[i] = v;
+}
+
+struct RsdArgs {
+  int N;
+  float H_z;
+  double cell;    // BoxSize / (float)N  [Mpc]
+  double half;    // (subcell_width / 2) / cell
+  double frac_w;  // subcell_width / cell
+  float x_pos[kRsdSteps];
+  float x_off[kRsdSteps];
+};
+
+// Line-of-sight re-binning of the sub-cells (:312-496), one thread per (x, y)
+// column, cell by cell and sub-cell by sub-cell as the reference does, so every
+// bin receives its additions in the reference's order.  rsd_cell places the
+// 50 sub-cells of cell k and hands each share to a sink: a float share
+// (`bin += delta_T / 50.f`) or a double one (`bin += fraction * delta_T /
+// 50.f`, rounded to float after the addition).  Indices the reference's single
+// +-N correction would leave outside the column are wrapped periodically.
+__device__ __forceinline__ int rsd_wrap(int k, int N) { return ((k % N) + N) % N; }
+
+template <class Sink>
+__device__ __forceinline__ void rsd_cell(int k, int N, float d, double d1_low, double d2_low, double d1_high,
+                                         double d2_high, const RsdArgs& a, Sink& sink)
+{
+  const float share = d / (float)kRsdSteps;
+  const double w = (double)(float)kRsdSteps;
+  for (int s = 0; s < kRsdSteps; ++s) {
+    const double xp = a.x_pos[s];
+    const double disp = (xp <= 0.5) ? d1_low + ((xp + 0.5) - 0.) * (d2_low - d1_low) / (1. - 0.)
+                                    : d1_high + ((xp - 0.5) - 0.) * (d2_high - d1_high) / (1. - 0.);
+    const double pos = ((double)a.x_off[s] + disp) / a.cell;
+    const double lo = pos - a.half, hi = pos + a.half;
+    if (lo >= 0.0 && hi < 1.0) {
+      sink.f(k, share);
+    } else if (lo < 0.0 && hi < 0.0) {
+      const double cd = ceil(fabs(lo)) - 1.;
+      const int c = (int)cd;
+      if (fabs(hi) > cd) {
+        sink.f(rsd_wrap(k < c + 1 ? k - (c + 1) + N : k - (c + 1), N), share);
+      } else {
+        const double fo = (fabs(lo) - cd) / a.frac_w, fw = 1. - fo;
+        sink.d(rsd_wrap(k < c ? k - c + N : k - c, N), fw * (double)d / w);
+        sink.d(rsd_wrap(k < c + 1 ? k - (c + 1) + N : k - (c + 1), N), fo * (double)d / w);
+      }
+    } else if (lo < 0.0 && (hi > 0.0 && hi < 1.0)) {
+      const double fw = hi / a.frac_w, fo = 1. - fw;
+      sink.d(k == 0 ? N - 1 : k - 1, fo * (double)d / w);
+      sink.d(k, fw * (double)d / w);
+    } else if ((lo >= 0.0 && lo < 1.0) && hi >= 1.0) {
+      const double fo = (hi - 1.) / a.frac_w, fw = 1. - fo;
+      sink.d(k, fw * (double)d / w);
+      sink.d(k == N - 1 ? 0 : k + 1, fo * (double)d / w);
+    } else {
+      const double cd = floor(fabs(hi));
+      const int c = (int)cd;
+      if (lo >= cd) {
+        sink.f(rsd_wrap(k > N - 1 - c ? k + c - N : k + c, N), share);
+      } else {
+        const double fo = (hi - cd) / a.frac_w, fw = 1. - fo;
+        sink.d(rsd_wrap(k > N - 1 - (c - 1) ? k + c - 1 - N : k + c - 1, N), fw * (double)d / w);
+        sink.d(rsd_wrap(k > N - 1 - c ? k + c - N : k + c, N), fo * (double)d / w);
+      }
+    }
+  }
+}
+
+// Bins of the column in global memory, bins[k * N^2 + column] (coalesced across a wave).
+struct RsdGlobalSink {
+  float* b;
+  long NN;
+  __device__ __forceinline__ void f(int k, float v)
+  {
+    float* q = b + (long)k * NN;
+    *q = *q + v;
+  }
+  __device__ __forceinline__ void d(int k, double v)
+  {
+    float* q = b + (long)k * NN;
+    *q = (float)((double)*q + v);
+  }
+};
+
+__global__ __launch_bounds__(kNT) void k_rsd_los(float* __restrict__ dT, const float* __restrict__ xH,
+                                                const float* __restrict__ vel, float* __restrict__ bins, RsdArgs a)
+{
+  const long col = (long)blockIdx.x * kNT + threadIdx.x;
+  const int N = a.N;
+  const long NN = (long)N * N;
+  if (col >= NN) return;
+  const float* dt = dT + col * N;
+  const float* xh = xH + col * N;
+  const float* vl = vel + col * (N + 2);
+  float* o = dT + col * N;
+  float* b = bins + col;
+  for (int k = 0; k < N; ++k) b[k * NN] = 0.0f;
+  RsdGlobalSink g{b, NN};
+  for (int k = 0; k < N; ++k) {
+    const float d = dt[k];
+    if (!(fabs((double)d) >= 1e-7 && (double)xh[k] >= 1e-7)) continue;
+    const int km = (k == 0) ? N - 1 : k - 1, kp = (k == N - 1) ? 0 : k + 1;
+    rsd_cell(k, N, d, vl[km] / a.H_z, vl[k] / a.H_z, vl[k] / a.H_z, vl[kp] / a.H_z, a, g);
+  }
+  for (int k = 0; k < N; ++k) o[k] = b[k * NN];
+}
+#pragma clang fp contract(on)
+
 }  // namespace mrx

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .abi import GridParams, HiiParams, TbParams, TsParams
+from .abi import GridParams, HiiParams, TbParams, TbVelParams, TsParams
 
 # src/meraxes.h:21-44
 GRAVITY = 6.672e-8
@@ -103,6 +103,18 @@ def tb_params(grid_dim, redshift, include_spin_temp=0, on_device=0):
     p.hubble_h = HUBBLE_H
     p.omega_m = OMEGA_M
     p.baryon_frac = BARYON_FRAC
+    return p
+
+
+def tb_vel_params(grid_dim, box_size, redshift, pec_vels=3, trees_id=1, on_device=0):
+    """mrx_tb_vel_params: the spin-temperature delta_T with peculiar velocities (Tiamat cosmology)."""
+    p = TbVelParams()
+    p.tb = tb_params(grid_dim, redshift, include_spin_temp=1, on_device=on_device)
+    p.pec_vels = pec_vels
+    p.trees_id = trees_id
+    p.box_size = box_size
+    p.omega_lambda = OMEGA_L
+    p.omega_r = 0.0
     return p
 
 

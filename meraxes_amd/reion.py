@@ -188,6 +188,24 @@ def brightness_temp(ctx, params, xH, deltax, TS_box=None, out=None):
     return out, ave.value
 
 
+def brightness_temp_vel(ctx, params, xH, deltax, TS_box, vel, out=None):
+    """delta_T with peculiar velocities (BrightnessTemperature.c:126-510).
+
+    ``params`` is an mrx_tb_vel_params; ``vel`` (padded) is converted in place
+    to comoving velocity, as the reference does.  Returns (delta_T, <delta_T>).
+    """
+    N = params.tb.grid_dim
+    on_dev = _is_dev(xH)
+    params.tb.pointers_on_device = 1 if on_dev else 0
+    if out is None:
+        out = _empty_like_real(xH, N)
+    ave = C.c_double()
+    _bind_stream(ctx, on_dev)
+    ctx.check(ctx.lib.mrx_brightness_temp_vel(ctx.ptr, C.byref(params), _ptr(xH), _ptr(deltax), _ptr(TS_box),
+                                              _ptr(vel), _ptr(out), C.byref(ave)), "brightness_temp_vel")
+    return out, ave.value
+
+
 def profile(ctx, enable=True):
     """Start (or stop) per-kernel event timing on the context's stream."""
     ctx.check(ctx.lib.mrx_profile(ctx.ptr, 1 if enable else 0), "profile")

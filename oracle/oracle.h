@@ -88,6 +88,13 @@ int orc_construct_baryon_grids(const mrx_grid_params* p, long n_gal, const float
                                const double* fesc, const double* gross, const double* bhm, const double* ebhm,
                                float* stars, float* wsfr, float* sfr, float* hist, long* stats);
 
+/* ComputeBrightnessTemperatureBox with peculiar velocities
+ * (BrightnessTemperature.c:126-510); vel (padded) is converted in place. */
+int orc_brightness_temp_vel(const mrx_tb_vel_params* p, const float* xH, const float* deltax, const float* TS_box,
+                            float* vel, float* delta_T, double* ave_Tb);
+/* Its line-of-sight re-binning alone (:312-496): delta_T in place, vel converted. */
+int orc_rsd_los(const mrx_tb_vel_params* p, const float* xH, const float* vel, float* delta_T);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,7 @@ import subprocess
 
 import numpy as np
 
-from meraxes_amd.abi import GridParams, HiiGlobals, HiiParams, TbParams, TsGlobals, TsParams, fptr
+from meraxes_amd.abi import GridParams, HiiGlobals, HiiParams, TbParams, TbVelParams, TsGlobals, TsParams, fptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -48,6 +48,8 @@ def lib():
         L.orc_compute_ts.argtypes = [C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals), vp]
         L.orc_ts_export.argtypes = [C.POINTER(TsParams), vp, i]
         L.orc_construct_baryon_grids.argtypes = [C.POINTER(GridParams), C.c_long] + [vp] * 11
+        L.orc_brightness_temp_vel.argtypes = [C.POINTER(TbVelParams)] + [vp] * 5 + [C.POINTER(C.c_double)]
+        L.orc_rsd_los.argtypes = [C.POINTER(TbVelParams), vp, vp, vp]
         L.orc_set_threads.argtypes = [i]
         L.orc_get_threads.restype = i
         _lib = L
@@ -175,6 +177,26 @@ def brightness_temp(p, xH, deltax, TS_box=None):
     args = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None for a in (xH, deltax, TS_box)]
     _chk(lib().orc_brightness_temp(C.byref(p), *[fptr(a) for a in args], fptr(out), C.byref(ave)), "tb")
     return out, ave.value
+
+
+def brightness_temp_vel(p, xH, deltax, TS_box, vel):
+    """delta_T with peculiar velocities; returns (delta_T, <delta_T>, converted vel) - vel is copied."""
+    N = p.tb.grid_dim
+    out = np.empty((N, N, N), np.float32)
+    ave = C.c_double()
+    v = np.array(vel, np.float32, copy=True)
+    args = [np.ascontiguousarray(a, dtype=np.float32) for a in (xH, deltax, TS_box)]
+    _chk(lib().orc_brightness_temp_vel(C.byref(p), *[fptr(a) for a in args], fptr(v), fptr(out), C.byref(ave)),
+         "tb_vel")
+    return out, ave.value, v
+
+
+def rsd_los(p, xH, vel_converted, delta_T):
+    """The sub-cell re-binning alone on a delta_T grid (returns a new grid)."""
+    out = np.array(delta_T, np.float32, copy=True)
+    _chk(lib().orc_rsd_los(C.byref(p), fptr(np.ascontiguousarray(xH, np.float32)),
+                           fptr(np.ascontiguousarray(vel_converted, np.float32)), fptr(out)), "rsd_los")
+    return out
 
 
 def compute_ts(p, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box, want_shells=False):
