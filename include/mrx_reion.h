@@ -64,6 +64,15 @@ typedef struct mrx_hii_params {
   double unit_length_cm;     /* units.UnitLength_in_cm                                      */
   double unit_mass_g;        /* units.UnitMass_in_g                                         */
   double unit_time_s;        /* units.UnitTime_in_s                                         */
+  /* inhomogeneous recombinations (mrx_find_hii_bubbles_rec only; r_bubble_max is
+   * then physics.ReionRBubbleMaxRecomb, find_HII_bubbles.c:186-190) */
+  int include_recombinations;/* Flag_IncludeRecombinations                                   */
+  int pad_;                  /* unused, 0                                                    */
+  double prev_redshift;      /* ZZ[snapshot-1] (ZZ[0] at snapshot 0): zstep (:83-90)         */
+  double hubble_h;           /* Hubble_h                                                     */
+  double omega_lambda;       /* OmegaLambda (dtdz, XRayHeatingFunctions.c:377-392)           */
+  double baryon_frac;        /* BaryonFrac } the hydrogen density No (reionization.h:17-27)  */
+  double y_he;               /* physics.Y_He }                                               */
 } mrx_hii_params;
 
 /* Scalar results the reference stores in run_globals.reion_grids (:487-494). */
@@ -168,11 +177,26 @@ int mrx_radius_ladder(const mrx_hii_params* p, double* R_out, int max_n);
  * in/out (state carried across snapshots): z_at_ionization, and
  *      J_21_at_ionization (only with p->uvb_flag)
  * globals: volume/mass weighted xH and volume weighted J_21, as the reference.
- * Inhomogeneous recombinations and minihalos are not implemented.
+ * p->include_recombinations must be 0 here (see mrx_find_hii_bubbles_rec);
+ * minihalos are not implemented.
  */
 int mrx_find_hii_bubbles(mrx_ctx* ctx, const mrx_hii_params* p, const float* deltax, const float* stars,
                          const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
                          float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out);
+
+/* The same with inhomogeneous recombinations (Flag_IncludeRecombinations,
+ * find_HII_bubbles.c:155-163, :246-302, :325-404, :472-477): N_rec (padded,
+ * in/out) is filtered as one more field, a cell ionises when
+ * f_coll*eps > n_f (1 + N_rec_R / (1 + delta_R)), Gamma12 (unpadded, in/out)
+ * takes Gamma_R_prefactor(R) * sfr_R / V at a cell's first crossing, and after
+ * the ladder N_rec += RR(z_eff, Gamma12) |dt/dz| zstep (1 - xH) with the
+ * splined MHR00 recombination rate (recombinations.c).  With
+ * p->include_recombinations = 0, N_rec and Gamma12 may be NULL and the call is
+ * mrx_find_hii_bubbles. */
+int mrx_find_hii_bubbles_rec(mrx_ctx* ctx, const mrx_hii_params* p, const float* deltax, const float* stars,
+                             const float* weighted_sfr, const float* x_e, float* N_rec, float* xH, float* r_bubble,
+                             float* J_21, float* z_at_ionization, float* J_21_at_ionization, float* Gamma12,
+                             mrx_hii_globals* out);
 
 /* ---------------------------------------------------------------------------
  * filter().  Drop-in for filter(box, local_ix_start, slab_nx, grid_dim, R,

@@ -19,7 +19,7 @@ EXPORTS = (
     "mrx_brightness_temp", "mrx_abi_sizeof", "mrx_profile", "mrx_profile_read", "mrx_compute_ts",
     "mrx_hii_batch_create", "mrx_hii_batch_run", "mrx_hii_batch_describe", "mrx_hii_batch_free",
     "mrx_power_spectrum", "mrx_power_spectrum_adjoint", "mrx_snapshot_adjoint", "mrx_construct_baryon_grids",
-    "mrx_brightness_temp_vel",
+    "mrx_brightness_temp_vel", "mrx_find_hii_bubbles_rec",
 )
 
 _lib = None
@@ -66,6 +66,7 @@ def load():
     L.mrx_snapshot_adjoint.argtypes = [ctxp, vp, C.POINTER(HiiParams), C.POINTER(TbParams)] + [vp] * 16
     L.mrx_construct_baryon_grids.argtypes = ([ctxp, C.POINTER(GridParams), C.c_long] + [vp] * 10 +
                                              [C.POINTER(GridStats)])
+    L.mrx_find_hii_bubbles_rec.argtypes = [ctxp, C.POINTER(HiiParams)] + [vp] * 11 + [C.POINTER(HiiGlobals)]
     L.mrx_brightness_temp_vel.argtypes = [ctxp, C.POINTER(TbVelParams)] + [vp] * 5 + [C.POINTER(C.c_double)]
     L.mrx_abi_sizeof.argtypes = [i]
     L.mrx_abi_sizeof.restype = C.c_size_t

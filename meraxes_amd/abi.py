@@ -50,6 +50,13 @@ class HiiParams(C.Structure):
         ("unit_length_cm", C.c_double),
         ("unit_mass_g", C.c_double),
         ("unit_time_s", C.c_double),
+        ("include_recombinations", C.c_int),
+        ("pad_", C.c_int),
+        ("prev_redshift", C.c_double),
+        ("hubble_h", C.c_double),
+        ("omega_lambda", C.c_double),
+        ("baryon_frac", C.c_double),
+        ("y_he", C.c_double),
     ]
 
 

@@ -35,18 +35,21 @@ def build(verbose=False):
     src = os.path.join(CSRC, "mrx_reion.hip")
     tmp = OUT + ".tmp"
     # host-side spin-temperature tables: plain C, no fused multiply-adds
-    obj = os.path.join(CSRC, "ts_tables.o")
-    ccmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-c",
-            os.path.join(CSRC, "ts_tables.c"), "-o", obj]
-    if verbose:
-        print(" ".join(ccmd), flush=True)
-    r = subprocess.run(ccmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"gcc failed:\n{r.stderr}")
+    objs = []
+    for name in ("ts_tables", "recomb_tables"):  # host tables of the spin temperature and the recombinations
+        obj = os.path.join(CSRC, name + ".o")
+        ccmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-c",
+                os.path.join(CSRC, name + ".c"), "-o", obj]
+        if verbose:
+            print(" ".join(ccmd), flush=True)
+        r = subprocess.run(ccmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"gcc failed:\n{r.stderr}")
+        objs.append(obj)
     # the HIP translation unit and the C object are linked together through
     # the linker so hipcc does not treat the object as a HIP source
     grid = os.path.join(CSRC, "grid.hip")  # galaxy gridding (rocPRIM sort): its own translation unit
-    cmd = [hipcc()] + flags() + ["-o", tmp, src, grid, "-Xlinker", obj, "-lpthread"]
+    cmd = [hipcc()] + flags() + ["-o", tmp, src, grid] + [a for o in objs for a in ("-Xlinker", o)] + ["-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -23,6 +23,7 @@
 #include "ts_kernels.hpp"
 #include "adjoint.hpp"
 #include "ts_tables.h"
+#include "recomb_tables.h"
 
 using namespace mrx;
 
@@ -30,6 +31,7 @@ namespace {
 // constants of src/meraxes.h:21-44 and reionization.h:38
 constexpr double kPlanck = 6.6262e-27;
 constexpr double kHubble = 3.2407789e-18;
+constexpr double kSigmaHI = 6.3e-18;  // SIGMA_HI, reionization.h:11
 constexpr double kProtonMass = 1.6726e-24;
 constexpr double kTcmb = 2.728;
 constexpr double kLFactor = 0.620350491;
@@ -84,6 +86,9 @@ struct mrx_ctx {
   size_t adj_work_cap = 0;
   double* adj_gS = nullptr;
   size_t adj_gS_cap = 0;
+  // recombination-rate tables (host, built once per No and Y_He) and their device copy
+  mrx_rr_tables* rr = nullptr;
+  RRDev* rr_dev = nullptr;
   // brightness temperature with velocities: gradient grid + line-of-sight bins
   float* tbv = nullptr;
   size_t tbv_cap = 0;
@@ -412,18 +417,25 @@ struct Pipe {
     return check_launch(ctx, "k_c2r_z");
   }
 
+  // the ladder's last pass; fields deltax, stars, sfr (+ x_e) (+ N_rec when rec)
   static int threshold(mrx_ctx* ctx, const float* deltax, float* xH, float* rb, float* J21, float* zin, float* J21i,
-                       const StepArgs& a)
+                       float* Gamma12, bool rec, const StepArgs& a)
   {
     const int nb = N * N / ThrCfg<N>::RZ;
     constexpr int NT = ThrCfg<N>::NT;
     PROF("c2r_threshold");
-    if (a.nfield == 4)
-      k_c2r_threshold<N, 4><<<nb, NT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, zin,
-                                                        J21i, ctx->Rtab, ctx->partials, a);
+#define MRX_THR(NF_, REC_)                                                                                          \
+  k_c2r_threshold<N, NF_, REC_><<<nb, NT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, \
+                                                           zin, J21i, Gamma12, ctx->Rtab, ctx->partials, a)
+    if (!rec && a.nfield == 4)
+      MRX_THR(4, false);
+    else if (!rec)
+      MRX_THR(3, false);
+    else if (a.nfield == 5)
+      MRX_THR(5, true);
     else
-      k_c2r_threshold<N, 3><<<nb, NT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, zin,
-                                                        J21i, ctx->Rtab, ctx->partials, a);
+      MRX_THR(4, true);
+#undef MRX_THR
     return check_launch(ctx, "k_c2r_threshold");
   }
 
@@ -512,15 +524,46 @@ int validate_hii(mrx_ctx* ctx, const mrx_hii_params* p)
   return MRX_OK;
 }
 
+// The recombination-rate table for the cosmology of p, on the device.
+int rr_tables(mrx_ctx* ctx, const mrx_hii_params* p, const RRDev** out)
+{
+  const double No = mrx_rr_No(p->hubble_h, p->omega_m, p->baryon_frac, p->y_he);
+  if (!ctx->rr || ctx->rr->No != No || ctx->rr->y_he != p->y_he) {
+    if (!ctx->rr) ctx->rr = new mrx_rr_tables();
+    if (mrx_rr_init(ctx->rr, No, p->y_he) != 0) {
+      delete ctx->rr;
+      ctx->rr = nullptr;
+      return set_err(ctx, MRX_EINVAL, "recombination-rate tables failed");
+    }
+    if (!ctx->rr_dev) HIP_TRY(hipMalloc(&ctx->rr_dev, sizeof(RRDev)));
+    static_assert(sizeof(RRDev) == sizeof(double) * (kRRG + 4 * kRRZ * kRRG), "RRDev layout");
+    static_assert(kRRZ == MRX_RR_Z_NPTS && kRRG == MRX_RR_LNG_NPTS, "table sizes");
+    std::vector<double> h(kRRG + 4 * (size_t)kRRZ * kRRG);
+    memcpy(h.data(), ctx->rr->lnG, sizeof(double) * kRRG);
+    const size_t row = (size_t)kRRZ * kRRG;
+    memcpy(h.data() + kRRG, ctx->rr->y, sizeof(double) * row);
+    memcpy(h.data() + kRRG + row, ctx->rr->b, sizeof(double) * row);
+    memcpy(h.data() + kRRG + 2 * row, ctx->rr->c, sizeof(double) * row);
+    memcpy(h.data() + kRRG + 3 * row, ctx->rr->d, sizeof(double) * row);
+    HIP_TRY(hipMemcpy(ctx->rr_dev, h.data(), sizeof(RRDev), hipMemcpyHostToDevice));
+  }
+  *out = ctx->rr_dev;
+  return MRX_OK;
+}
+
 struct HiiRun {
   template <int N>
   struct At {
+    // in: deltax, stars, weighted_sfr, x_e, N_rec; out: xH, r_bubble, J_21,
+    // z_at_ionization, J_21_at_ionization, Gamma12 (the last in and out only
+    // with recombinations)
     static int run(mrx_ctx* ctx, const mrx_hii_params* p, const float* const* in, float* const* out,
                    mrx_hii_globals* g)
     {
       using P = Pipe<N>;
-      const int nfield = p->include_spin_temp ? 4 : 3;
-      MRX_TRY(ensure_buffers(ctx, N, 4));
+      const bool rec = p->include_recombinations != 0;
+      const int nfield = 3 + (p->include_spin_temp ? 1 : 0) + (rec ? 1 : 0);
+      MRX_TRY(ensure_buffers(ctx, N, nfield > 4 ? nfield : 4));
       std::vector<double> R;
       const int S = ladder(p, R);
       if (S < 1) return set_err(ctx, MRX_EINVAL, "radius ladder is empty");
@@ -536,9 +579,10 @@ struct HiiRun {
       // fixed stride when staged; device pointers are used where they lie.
       const long fs = (long)npad(N);
       const float* deltax = in[0];
+      const float* src[5] = {in[0], in[1], in[2], p->include_spin_temp ? in[3] : in[4], in[4]};
       for (int f = 0; f < nfield; ++f) {
         // forward transform of each field into its U slot
-        MRX_TRY(P::forward(ctx, in[f], 0, 1, true, ctx->U + (long)f * P::nc, ctx->V + (long)f * N * N));
+        MRX_TRY(P::forward(ctx, src[f], 0, 1, true, ctx->U + (long)f * P::nc, ctx->V + (long)f * N * N));
       }
       (void)fs;
       const double box_size = p->box_size;
@@ -561,9 +605,34 @@ struct HiiRun {
         a.M_mean = RtoM(p, R[s]);
         a.R_cubed = R[s] * R[s] * R[s];
         a.kfc = (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume / a.M_mean;
+        if (rec) {
+          // Gamma_R_prefactor (find_HII_bubbles.c:325-334): R h^-1 to R, pixel volume (Mpc/h)^3 to cm^3
+          double gp = (1.0 + z) * (1.0 + z) * R[s] * (p->unit_length_cm / p->hubble_h) * kSigmaHI * p->alpha_uv /
+                      (p->alpha_uv + 2.75) / 1.0e-12;
+          gp *= (p->unit_mass_g / p->unit_time_s) * pow(p->unit_length_cm / p->hubble_h, -3.) *
+                p->nion_phot_per_bary / kProtonMass;
+          a.gamma_pref = gp;
+        } else {
+          a.gamma_pref = 0.0;
+        }
         a.step = s;
         a.last = last ? 1 : 0;
-        MRX_TRY(P::threshold(ctx, deltax, out[0], out[1], out[2], out[3], out[4], a));
+        MRX_TRY(P::threshold(ctx, deltax, out[0], out[1], out[2], out[3], out[4], out[5], rec, a));
+      }
+      if (rec) {
+        // N_rec += dNrec after the ladder (:472-477)
+        const RRDev* rr;
+        MRX_TRY(rr_tables(ctx, p, &rr));
+        const float zstep = (float)(p->prev_redshift - p->redshift);
+        const float fabs_dtdz =
+          (float)fabs(mrx_cosmo_dtdz(p->omega_lambda, p->omega_m, p->hubble_h, (float)p->redshift) / p->hubble_h);
+        const long nr = (long)N * N * N;
+        {
+          PROF("nrec_update");
+          k_nrec_update<<<(unsigned)((nr + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(
+            const_cast<float*>(in[4]), out[5], out[0], deltax, rr, N, p->redshift, fabs_dtdz, zstep);
+        }
+        MRX_TRY(check_launch(ctx, "k_nrec_update"));
       }
       const int nb = P::nblocks_threshold();
       {
@@ -1342,6 +1411,8 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->adj_work);
   dfree(ctx->adj_gS);
   dfree(ctx->tbv);
+  dfree(ctx->rr_dev);
+  delete ctx->rr;
   for (auto& e : ctx->ext) dfree(e);
   delete ctx->tsd;
   delete ctx->tss;
@@ -1384,9 +1455,10 @@ int mrx_radius_ladder(const mrx_hii_params* p, double* R_out, int max_n)
   return n;
 }
 
-int mrx_find_hii_bubbles(mrx_ctx* ctx, const mrx_hii_params* p, const float* deltax, const float* stars,
-                         const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
-                         float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out)
+int mrx_find_hii_bubbles_rec(mrx_ctx* ctx, const mrx_hii_params* p, const float* deltax, const float* stars,
+                             const float* weighted_sfr, const float* x_e, float* N_rec, float* xH, float* r_bubble,
+                             float* J_21, float* z_at_ionization, float* J_21_at_ionization, float* Gamma12,
+                             mrx_hii_globals* out)
 {
   if (!ctx) return MRX_EINVAL;
   MRX_TRY(validate_hii(ctx, p));
@@ -1395,42 +1467,58 @@ int mrx_find_hii_bubbles(mrx_ctx* ctx, const mrx_hii_params* p, const float* del
   if (p->uvb_flag && (!J_21 || !J_21_at_ionization))
     return set_err(ctx, MRX_EINVAL, "uvb_flag needs J_21 and J_21_at_ionization");
   if (p->include_spin_temp && !x_e) return set_err(ctx, MRX_EINVAL, "include_spin_temp needs x_e");
+  const bool rec = p->include_recombinations != 0;
+  if (rec && (!N_rec || !Gamma12)) return set_err(ctx, MRX_EINVAL, "include_recombinations needs N_rec and Gamma12");
+  if (rec && !(p->hubble_h > 0 && p->y_he > 0 && p->y_he < 1 && p->baryon_frac > 0 && p->omega_lambda > 0))
+    return set_err(ctx, MRX_EINVAL, "include_recombinations needs hubble_h, y_he, baryon_frac and omega_lambda");
   HIP_TRY(hipSetDevice(ctx->device));
   const int N = p->grid_dim;
   const size_t np = npad(N), nr = nreal(N);
-  const float* in[4] = {deltax, stars, weighted_sfr, x_e};
-  float* outp[5] = {xH, r_bubble, J_21, z_at_ionization, J_21_at_ionization};
+  const float* in[5] = {deltax, stars, weighted_sfr, x_e, rec ? N_rec : nullptr};
+  float* outp[6] = {xH, r_bubble, J_21, z_at_ionization, J_21_at_ionization, rec ? Gamma12 : nullptr};
   if (p->pointers_on_device) {
     // the kernels move four cells per 16-byte access
     for (const float* q : {deltax, stars, weighted_sfr, x_e, (const float*)xH, (const float*)r_bubble,
-                           (const float*)J_21, (const float*)z_at_ionization, (const float*)J_21_at_ionization})
+                           (const float*)J_21, (const float*)z_at_ionization, (const float*)J_21_at_ionization,
+                           (const float*)in[4], (const float*)outp[5]})
       if (((uintptr_t)q & 15u) != 0) return set_err(ctx, MRX_EINVAL, "device grids must be 16-byte aligned");
+    MRX_TRY(dispatch<HiiRun>(N, ctx, p, in, outp, out));
+    return finish(ctx);
   }
-  if (!p->pointers_on_device) {
-    // stage: 4 padded inputs + 5 unpadded outputs
-    MRX_TRY(ensure_stage(ctx, 4 * np + 5 * nr));
-    float* d_in[4];
-    float* d_out[5];
-    for (int f = 0; f < 4; ++f) d_in[f] = ctx->stage + f * np;
-    for (int o = 0; o < 5; ++o) d_out[o] = ctx->stage + 4 * np + o * nr;
-    const int nfield = p->include_spin_temp ? 4 : 3;
-    for (int f = 0; f < nfield; ++f)
-      HIP_TRY(hipMemcpyAsync(d_in[f], in[f], sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d_out[3], z_at_ionization, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
-    if (p->uvb_flag)
-      HIP_TRY(hipMemcpyAsync(d_out[4], J_21_at_ionization, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
-    const float* din_c[4] = {d_in[0], d_in[1], d_in[2], nfield == 4 ? d_in[3] : nullptr};
-    MRX_TRY(dispatch<HiiRun>(N, ctx, p, din_c, d_out, out));
-    const int nout = p->uvb_flag ? 5 : 4;
-    for (int o = 0; o < nout; ++o) {
-      if (o == 2 && !p->uvb_flag) continue;
-      HIP_TRY(hipMemcpyAsync(outp[o], d_out[o], sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return MRX_OK;
+  // stage: 5 padded inputs + 6 unpadded outputs
+  MRX_TRY(ensure_stage(ctx, 5 * np + 6 * nr));
+  float* d_in[5];
+  float* d_out[6];
+  for (int f = 0; f < 5; ++f) d_in[f] = ctx->stage + f * np;
+  for (int o = 0; o < 6; ++o) d_out[o] = ctx->stage + 5 * np + o * nr;
+  for (int f = 0; f < 5; ++f)
+    if (in[f]) HIP_TRY(hipMemcpyAsync(d_in[f], in[f], sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(d_out[3], z_at_ionization, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+  if (p->uvb_flag)
+    HIP_TRY(hipMemcpyAsync(d_out[4], J_21_at_ionization, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+  if (rec) HIP_TRY(hipMemcpyAsync(d_out[5], Gamma12, sizeof(float) * nr, hipMemcpyHostToDevice, ctx->stream));
+  const float* din_c[5] = {d_in[0], d_in[1], d_in[2], in[3] ? d_in[3] : nullptr, rec ? d_in[4] : nullptr};
+  float* dout_c[6] = {d_out[0], d_out[1], d_out[2], d_out[3], d_out[4], rec ? d_out[5] : nullptr};
+  MRX_TRY(dispatch<HiiRun>(N, ctx, p, din_c, dout_c, out));
+  for (int o = 0; o < 6; ++o) {
+    if ((o == 2 || o == 4) && !p->uvb_flag) continue;
+    if (o == 5 && !rec) continue;
+    HIP_TRY(hipMemcpyAsync(outp[o], d_out[o], sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
   }
-  MRX_TRY(dispatch<HiiRun>(N, ctx, p, in, outp, out));
-  return finish(ctx);
+  if (rec) HIP_TRY(hipMemcpyAsync(N_rec, d_in[4], sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
+int mrx_find_hii_bubbles(mrx_ctx* ctx, const mrx_hii_params* p, const float* deltax, const float* stars,
+                         const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
+                         float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (p && p->include_recombinations)
+    return set_err(ctx, MRX_EINVAL, "include_recombinations: call mrx_find_hii_bubbles_rec with N_rec and Gamma12");
+  return mrx_find_hii_bubbles_rec(ctx, p, deltax, stars, weighted_sfr, x_e, nullptr, xH, r_bubble, J_21,
+                                  z_at_ionization, J_21_at_ionization, nullptr, out);
 }
 
 int mrx_filter(mrx_ctx* ctx, float* box, int local_ix_start, int slab_nx, int grid_dim, float R, int filter_type,
@@ -1813,6 +1901,8 @@ int mrx_hii_batch_create(mrx_ctx* ctx, const mrx_hii_batch_box* boxes, int n_box
   for (int i = 0; i < n_boxes; ++i) {
     const mrx_hii_batch_box& b = boxes[i];
     MRX_TRY(validate_hii(ctx, &b.params));
+    if (b.params.include_recombinations)
+      return set_err(ctx, MRX_EUNSUPPORTED, "batch: box %d: recombinations are not batched (mrx_find_hii_bubbles_rec)", i);
     if (!b.params.pointers_on_device)
       return set_err(ctx, MRX_EINVAL, "batch: box %d: grids must be device pointers", i);
     if (!b.deltax || !b.stars || !b.weighted_sfr || !b.xH || !b.r_bubble || !b.z_at_ionization)

@@ -649,6 +649,7 @@ struct StepArgs {
   double redshift;
   double bias;          // ReionGammaHaloBias
   double kfc;           // (4/3) pi R^3 / (pixel_volume M_mean): f_coll = stars * kfc / (1 + delta)
+  double gamma_pref;    // Gamma_R_prefactor(R) of the recombinations (:325-334)
   int step;             // ladder index (0 = largest R)
   int last;             // unfiltered final step
   int uvb;
@@ -682,19 +683,24 @@ struct ThrCfg {
   static_assert((N * N) % RZ == 0, "threshold workgroups must tile the rows");
 };
 
-template <int N, int NF>
+// Fields in T: deltax, stars, weighted_sfr, then x_e when XE (NF - REC = 4),
+// then N_rec when REC (inhomogeneous recombinations).
+template <int N, int NF, bool REC>
 __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T, long fstride,
                                                    const float2* __restrict__ twN,
                                                    const float* __restrict__ deltax_in,
                                                    uint8_t* __restrict__ state, float* __restrict__ xH,
                                                    float* __restrict__ r_bubble, float* __restrict__ J21,
                                                    float* __restrict__ z_in, float* __restrict__ J21_ion,
-                                                   const float* __restrict__ Rtab, double* __restrict__ partials,
-                                                   const StepArgs& a)
+                                                   float* __restrict__ Gamma12, const float* __restrict__ Rtab,
+                                                   double* __restrict__ partials, const StepArgs& a)
 {
 #pragma clang fp contract(off)
   using TC = ThrCfg<N>;
   constexpr int NH = TC::NH, RZ = TC::RZ, ZE = TC::ZE, NT = TC::NT, G = TC::G;
+  constexpr bool XE = (NF - (REC ? 1 : 0)) == 4;
+  constexpr int RI = XE ? 4 : 3;  // field of N_rec
+  static_assert(NF - (REC ? 1 : 0) == 3 || NF - (REC ? 1 : 0) == 4, "three or four fields besides N_rec");
   constexpr int TOT = NF * ZE;  // complex elements of every field's rows
   constexpr int EPT = TOT / NT;
   static_assert(TOT % NT == 0, "fields must split evenly over the threads");
@@ -720,22 +726,19 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   // consecutive cells of one row, so state is one word and the outputs are
   // 16-byte stores
   constexpr int CPT = 4 * G;
-  float val[4][CPT];
+  float val[NF][CPT];
 #pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    const int kind = (f == 0) ? 0 : (f == 3 ? 2 : 1);
+  for (int f = 0; f < NF; ++f) {
+    // clamps of :272-311: deltax, x_e, the rest (stars, sfr, N_rec)
+    const int kind = (f == 0) ? 0 : ((XE && f == 3) ? 2 : 1);
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      if (f < NF) {
-        const int h = f * ZE + 2 * (tid + j * NT);  // float2 index of the group's first pair
-        const float2 p0 = s[lds_addr<true>(h)], p1 = s[lds_addr<true>(h + 1)];
-        val[f][4 * j + 0] = clamp_kind(p0.x, kind);
-        val[f][4 * j + 1] = clamp_kind(p0.y, kind);
-        val[f][4 * j + 2] = clamp_kind(p1.x, kind);
-        val[f][4 * j + 3] = clamp_kind(p1.y, kind);
-      } else {
-        val[f][4 * j + 0] = val[f][4 * j + 1] = val[f][4 * j + 2] = val[f][4 * j + 3] = 0.0f;
-      }
+      const int h = f * ZE + 2 * (tid + j * NT);  // float2 index of the group's first pair
+      const float2 p0 = s[lds_addr<true>(h)], p1 = s[lds_addr<true>(h + 1)];
+      val[f][4 * j + 0] = clamp_kind(p0.x, kind);
+      val[f][4 * j + 1] = clamp_kind(p0.y, kind);
+      val[f][4 * j + 2] = clamp_kind(p1.x, kind);
+      val[f][4 * j + 3] = clamp_kind(p1.y, kind);
     }
   }
   double s_xH = 0.0, s_xHm = 0.0, s_m = 0.0, s_J = 0.0;
@@ -754,7 +757,9 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
       const int i = 4 * j + q;
       const long ir = ir0 + q;
       const double density_over_mean = 1.0 + (double)val[0][i];
-      const double neutral_fraction = a.use_xe ? 1.0 - (double)val[3][i] : 1.0;
+      const double neutral_fraction = XE ? 1.0 - (double)val[3][i] : 1.0;
+      // recombinations in the cell (:360-363)
+      const double rec = REC ? (double)val[RI][i] / density_over_mean : 0.0;
       // the reference's f_coll (four roundings of its exact value), evaluated
       // where needed: the last step's partial xH and near-threshold cells
       auto f_coll = [&]() {
@@ -765,14 +770,14 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
       // positive sides within 1e-14 of their exact values: a relative gap
       // above 1e-12 fixes the reference's outcome, closer cells take its
       // own expression
-      const double lhs = (double)val[1][i] * a.kfc * a.eff, rhs = neutral_fraction * density_over_mean;
+      const double lhs = (double)val[1][i] * a.kfc * a.eff, rhs = neutral_fraction * (1. + rec) * density_over_mean;
       bool cross;
       if (lhs > rhs * (1.0 + 1e-12))
         cross = true;
       else if (lhs < rhs * (1.0 - 1e-12))
         cross = false;
       else
-        cross = f_coll() * a.eff > neutral_fraction * (1. + 0.0);
+        cross = f_coll() * a.eff > neutral_fraction * (1. + rec);
       // J_21 of the cell at this radius (:317-320, :331-337)
       auto j21_aux = [&]() { return a.uvb ? (float)((double)val[2][i] / a.pixel_volume * a.J21c) : 0.0f; };
       const uint32_t st = (st_old >> (8 * q)) & 0xFFu;
@@ -781,6 +786,7 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
       if (cross) {
         if (never) {
           if (a.uvb) J21[ir] = j21_aux();
+          if (REC) Gamma12[ir] = (float)(a.gamma_pref * ((double)val[2][i] / a.pixel_volume));  // :398-404
           zcheck = true;
         }
         st_new = (st_new & ~(0xFFu << (8 * q))) | ((uint32_t)a.step << (8 * q));
@@ -855,18 +861,18 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   }
 }
 
-template <int N, int NF>
+template <int N, int NF, bool REC = false>
 __global__ __launch_bounds__(ThrCfg<N>::NT) void k_c2r_threshold(const float2* __restrict__ T, long fstride,
                                                       const float2* __restrict__ twN,
                                                       const float* __restrict__ deltax_in,
                                                       uint8_t* __restrict__ state, float* __restrict__ xH,
                                                       float* __restrict__ r_bubble, float* __restrict__ J21,
                                                       float* __restrict__ z_in, float* __restrict__ J21_ion,
-                                                      const float* __restrict__ Rtab, double* __restrict__ partials,
-                                                      StepArgs a)
+                                                      float* __restrict__ Gamma12, const float* __restrict__ Rtab,
+                                                      double* __restrict__ partials, StepArgs a)
 {
-  c2r_threshold_tile<N, NF>(T, fstride, twN, deltax_in, state, xH, r_bubble, J21, z_in, J21_ion, Rtab, partials,
-                            a);
+  c2r_threshold_tile<N, NF, REC>(T, fstride, twN, deltax_in, state, xH, r_bubble, J21, z_in, J21_ion, Gamma12, Rtab,
+                                 partials, a);
 }
 
 // Per-box grids and scalars of a batch (device array, one entry per box).
@@ -900,9 +906,9 @@ __global__ __launch_bounds__(ThrCfg<N>::NT) void k_c2r_threshold_batch(const flo
   a.redshift = bx.redshift;
   a.bias = bx.bias;
   a.J21c = J21c_tab[(long)box * nstep + a.step];
-  c2r_threshold_tile<N, NF>(T + box * bstride, fstride, twN, bx.deltax, state + (long)box * N * N * N, bx.xH,
-                        bx.r_bubble, bx.J21, bx.z_in, bx.J21_ion, Rtab,
-                        partials + (long)box * gridDim.x * 4, a);
+  c2r_threshold_tile<N, NF, false>(T + box * bstride, fstride, twN, bx.deltax, state + (long)box * N * N * N, bx.xH,
+                                   bx.r_bubble, bx.J21, bx.z_in, bx.J21_ion, nullptr, Rtab,
+                                   partials + (long)box * gridDim.x * 4, a);
 }
 
 // ------------------------------------------------- brightness temperature
@@ -938,6 +944,63 @@ __global__ __launch_bounds__(kNT) void k_brightness_temp(const float* __restrict
   acc = block_sum<kNT>(acc, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
+
+// ------------------------------------------------ recombinations (N_rec)
+#pragma clang fp contract(off)
+// Recombination-rate table on the device: knots in ln Gamma12 and the natural
+// cubic spline of each redshift row (recombinations.c:56-82).
+constexpr int kRRZ = 300, kRRG = 150;
+struct RRDev {
+  double lnG[kRRG];
+  double y[kRRZ][kRRG], b[kRRZ][kRRG], c[kRRZ][kRRG], d[kRRZ][kRRG];
+};
+
+// splined_recombination_rate (recombinations.c:30-54); ln Gamma12 above the
+// last knot is evaluated on the last interval (gsl_spline_eval would abort)
+__device__ __forceinline__ double rr_rate(const RRDev* __restrict__ t, float z_eff, double gamma12_bg)
+{
+  int z_ct = (int)(z_eff / 0.2f + 0.5);
+  double lnGamma = log(gamma12_bg);
+  if (z_ct < 0)
+    z_ct = 0;
+  else if (z_ct >= kRRZ)
+    z_ct = kRRZ - 1;
+  if (lnGamma < -10.0) return 0;
+  if (lnGamma >= (-10.0 + 0.1f * kRRG)) lnGamma = -10.0 + 0.1f * kRRG - 1e-7;
+  int lo = 0, hi = kRRG - 1;
+  if (lnGamma >= t->lnG[hi]) {
+    lo = hi - 1;
+  } else {
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (t->lnG[mid] > lnGamma)
+        hi = mid;
+      else
+        lo = mid;
+    }
+  }
+  const double dx = lnGamma - t->lnG[lo];
+  return t->y[z_ct][lo] + dx * (t->b[z_ct][lo] + dx * (t->c[z_ct][lo] + dx * t->d[z_ct][lo]));
+}
+
+// N_rec += RR(z_eff, Gamma12) |dt/dz| zstep (1 - xH) for every cell after the
+// ladder (find_HII_bubbles.c:472-477); deltax is the input grid.
+__global__ __launch_bounds__(kNT) void k_nrec_update(float* __restrict__ N_rec, const float* __restrict__ Gamma12,
+                                                    const float* __restrict__ xH,
+                                                    const float* __restrict__ deltax_in,
+                                                    const RRDev* __restrict__ rr, int N, double redshift,
+                                                    float fabs_dtdz, float zstep)
+{
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= (long)N * N * N) return;
+  const long ip = (i / N) * (N + 2) + i % N;
+  const double cell_xH = (double)xH[i];
+  const double density_over_mean = 1.0 + (double)deltax_in[ip];
+  const float z_eff = (float)((1. + redshift) * pow(density_over_mean, 1.0 / 3.0) - 1);
+  const double dNrec = rr_rate(rr, z_eff, (double)Gamma12[i]) * (double)fabs_dtdz * (double)zstep * (1. - cell_xH);
+  N_rec[ip] = (float)((double)N_rec[ip] + dNrec);
+}
+#pragma clang fp contract(on)
 
 // ----------------------------------- brightness temperature with velocities
 // Peculiar-velocity branch of ComputeBrightnessTemperatureBox

@@ -269,16 +269,18 @@ int mrx_ts_data_init(mrx_ts_data* d, const mrx_ts_params* p)
 
 /* ------------------------------------------------------- cosmology etc. */
 
-double mrx_ts_dtdz(const mrx_ts_data* d, float z)
+double mrx_cosmo_dtdz(double omega_l, double omega_m, double hubble_h, float z)
 {
-  const double OMl = d->omega_l, OMm = d->omega_m;
+  const double OMl = omega_l, OMm = omega_m;
   const double x = sqrt(OMl / OMm) * pow(1 + z, -3.0 / 2.0);
   const double dxdz = sqrt(OMl / OMm) * pow(1 + z, -5.0 / 2.0) * (-3.0 / 2.0);
-  const double const1 = 2 * sqrt(1 + OMm / OMl) / (3.0 * K_HUBBLE * d->hubble_h);
+  const double const1 = 2 * sqrt(1 + OMm / OMl) / (3.0 * K_HUBBLE * hubble_h);
   const double numer = dxdz * (1 + x * pow(pow(x, 2) + 1, -0.5));
   const double denom = x + sqrt(pow(x, 2) + 1);
   return (const1 * numer / denom);
 }
+
+double mrx_ts_dtdz(const mrx_ts_data* d, float z) { return mrx_cosmo_dtdz(d->omega_l, d->omega_m, d->hubble_h, z); }
 
 double mrx_ts_hubble(const mrx_ts_data* d, float z)
 {

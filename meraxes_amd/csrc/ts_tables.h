@@ -105,6 +105,9 @@ int mrx_ts_plan(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p);
  * x_e_ave of the first shell are known; writes stored_fcoll[snapshot]. */
 int mrx_ts_build_tables(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p);
 
+/* dt/dz of the flat LCDM cosmology (XRayHeatingFunctions.c:377-392) */
+double mrx_cosmo_dtdz(double omega_l, double omega_m, double hubble_h, float z);
+
 /* scalar physics, exposed for the tests */
 double mrx_ts_dtdz(const mrx_ts_data* d, float z);
 double mrx_ts_drdz(const mrx_ts_data* d, float z);

@@ -44,7 +44,7 @@ def units():
 
 def hii_params(grid_dim, box_size, redshift, filter_type=0, efficiency=20.0, uvb_flag=0,
                include_spin_temp=0, r_bubble_max=50.0, r_bubble_min=0.4068, delta_r_factor=1.1,
-               rtom_filter_type=0, on_device=0):
+               rtom_filter_type=0, on_device=0, include_recombinations=0, prev_redshift=None):
     """mrx_hii_params for one snapshot (defaults.par:221-251 for the unset knobs)."""
     u = units()
     p = HiiParams()
@@ -68,6 +68,13 @@ def hii_params(grid_dim, box_size, redshift, filter_type=0, efficiency=20.0, uvb
     p.unit_length_cm = UNIT_LENGTH_CM
     p.unit_mass_g = UNIT_MASS_G
     p.unit_time_s = u["unit_time_s"]
+    # recombinations (defaults.par:253 ReionRBubbleMaxRecomb = 33.9 when on)
+    p.include_recombinations = include_recombinations
+    p.prev_redshift = redshift if prev_redshift is None else prev_redshift
+    p.hubble_h = HUBBLE_H
+    p.omega_lambda = OMEGA_L
+    p.baryon_frac = BARYON_FRAC
+    p.y_he = Y_HE
     return p
 
 

@@ -90,12 +90,15 @@ def radius_ladder(params):
 
 
 def find_hii_bubbles(ctx, params, deltax, stars, weighted_sfr, x_e=None, z_at_ionization=None,
-                     J_21_at_ionization=None, out=None):
+                     J_21_at_ionization=None, out=None, N_rec=None, Gamma12=None):
     """One call of the bubble finder; returns the output grids and globals.
 
     ``z_at_ionization`` / ``J_21_at_ionization`` are the state carried between
     snapshots (updated in place when given; fresh -1 / 0 grids otherwise).
-    ``out`` may hold preallocated ``xH``, ``r_bubble``, ``J_21`` grids.
+    ``out`` may hold preallocated ``xH``, ``r_bubble``, ``J_21`` grids.  With
+    params.include_recombinations, ``N_rec`` (padded) and ``Gamma12`` are the
+    recombination state carried between snapshots, updated in place (N_rec is
+    required; Gamma12 defaults to a fresh zero grid).
     """
     N = params.grid_dim
     on_dev = _is_dev(deltax)
@@ -112,12 +115,24 @@ def find_hii_bubbles(ctx, params, deltax, stars, weighted_sfr, x_e=None, z_at_io
         z_at_ionization = _empty_like_real(deltax, N, -1.0)
     g = HiiGlobals()
     _bind_stream(ctx, on_dev)
-    ctx.check(ctx.lib.mrx_find_hii_bubbles(ctx.ptr, C.byref(params), _ptr(deltax), _ptr(stars), _ptr(weighted_sfr),
-                                           _ptr(x_e), _ptr(xH), _ptr(rb), _ptr(J21), _ptr(z_at_ionization),
-                                           _ptr(J_21_at_ionization), C.byref(g)), "find_hii_bubbles")
+    if params.include_recombinations:
+        if N_rec is None:
+            raise ValueError("include_recombinations needs the N_rec grid")
+        if Gamma12 is None:
+            Gamma12 = _empty_like_real(deltax, N, 0.0)
+        ctx.check(ctx.lib.mrx_find_hii_bubbles_rec(
+            ctx.ptr, C.byref(params), _ptr(deltax), _ptr(stars), _ptr(weighted_sfr), _ptr(x_e), _ptr(N_rec), _ptr(xH),
+            _ptr(rb), _ptr(J21), _ptr(z_at_ionization), _ptr(J_21_at_ionization), _ptr(Gamma12), C.byref(g)),
+            "find_hii_bubbles_rec")
+    else:
+        ctx.check(ctx.lib.mrx_find_hii_bubbles(ctx.ptr, C.byref(params), _ptr(deltax), _ptr(stars),
+                                               _ptr(weighted_sfr), _ptr(x_e), _ptr(xH), _ptr(rb), _ptr(J21),
+                                               _ptr(z_at_ionization), _ptr(J_21_at_ionization), C.byref(g)),
+                  "find_hii_bubbles")
     return {
         "xH": xH, "r_bubble": rb, "J_21": J21, "z_at_ionization": z_at_ionization,
         "J_21_at_ionization": J_21_at_ionization if params.uvb_flag else None,
+        "N_rec": N_rec, "Gamma12": Gamma12,
         "volume_weighted_global_xH": g.volume_weighted_global_xH,
         "mass_weighted_global_xH": g.mass_weighted_global_xH,
         "volume_weighted_global_J_21": g.volume_weighted_global_J_21, "n_radii": g.n_radii,

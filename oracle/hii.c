@@ -17,10 +17,12 @@
 #include <string.h>
 
 #include "oracle.h"
+#include "../meraxes_amd/csrc/recomb_tables.h"
 
 /* constants of src/meraxes.h:21-44 and reionization.h:38 */
 #define ORC_PLANCK 6.6262e-27
 #define ORC_PROTONMASS 1.6726e-24
+#define ORC_SIGMA_HI 6.3e-18 /* SIGMA_HI, reionization.h:11 */
 #define ORC_REL_TOL ((float)1e-5)
 #define ORC_ABS_TOL ((float)1e-8)
 #define ORC_L_FACTOR 0.620350491
@@ -272,11 +274,34 @@ static double orc_RtoM(const mrx_hii_params* p, double R)
   }
 }
 
+/* the recombination-rate table of the last cosmology asked for */
+static mrx_rr_tables* orc_rr(const mrx_hii_params* p)
+{
+  static mrx_rr_tables* t = NULL;
+  const double No = mrx_rr_No(p->hubble_h, p->omega_m, p->baryon_frac, p->y_he);
+  if (!t) t = (mrx_rr_tables*)calloc(1, sizeof *t);
+  if (t && (t->No != No || t->y_he != p->y_he))
+    if (mrx_rr_init(t, No, p->y_he) != 0) return NULL;
+  return t;
+}
+
 int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* deltax, const float* stars,
                          const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
                          float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out, float* max_ratio)
 {
+  if (p->include_recombinations) return MRX_EINVAL;
+  return orc_find_hii_bubbles_rec(precision, p, deltax, stars, weighted_sfr, x_e, NULL, xH, r_bubble, J_21,
+                                  z_at_ionization, J_21_at_ionization, NULL, out, max_ratio);
+}
+
+int orc_find_hii_bubbles_rec(int precision, const mrx_hii_params* p, const float* deltax, const float* stars,
+                             const float* weighted_sfr, const float* x_e, float* N_rec, float* xH, float* r_bubble,
+                             float* J_21, float* z_at_ionization, float* J_21_at_ionization, float* Gamma12,
+                             mrx_hii_globals* out, float* max_ratio)
+{
   const int N = p->grid_dim;
+  const int recomb = p->include_recombinations;
+  if (recomb && (!N_rec || !Gamma12)) return MRX_EINVAL;
   if (N < 2 || (N & 1) || !deltax || !stars || !weighted_sfr || !xH || !r_bubble || !z_at_ionization)
     return MRX_EINVAL;
   if (p->filter_type < 0 || p->filter_type > 2) return MRX_EFILTER;
@@ -303,11 +328,13 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
   if (max_ratio)
     for (long i = 0; i < nreal; i++) max_ratio[i] = 0.0f;
 
-  const int nf = p->include_spin_temp ? 4 : 3;
-  const float* src[4] = {deltax, stars, weighted_sfr, x_e};
-  const int clamp_kind[4] = {0, 1, 1, 2};
-  void* unf[4] = {0};
-  void* fil[4] = {0};
+  /* fields: deltax, stars, weighted_sfr, then x_e (spin temperature), then N_rec (:155-163) */
+  const int nf = 3 + (p->include_spin_temp ? 1 : 0) + (recomb ? 1 : 0);
+  const int fx = 3, fr = p->include_spin_temp ? 4 : 3;
+  const float* src[5] = {deltax, stars, weighted_sfr, p->include_spin_temp ? x_e : N_rec, N_rec};
+  const int clamp_kind[5] = {0, 1, 1, p->include_spin_temp ? 2 : 1, 1};
+  void* unf[5] = {0};
+  void* fil[5] = {0};
   for (int f = 0; f < nf; f++) {
     if (precision == 64) {
       unf[f] = orc_forward_f64(N, src[f]);
@@ -322,7 +349,6 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
   int S = orc_radius_ladder(p, R_ladder, 4096);
   if (S < 1 || S > 4096) return MRX_EINVAL;
 
-  const double rec = 0.0;
   for (int step = 0; step < S; step++) {
     const double R = R_ladder[step];
     const bool last = (step == S - 1);
@@ -352,25 +378,36 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
                                      p->unit_mass_g / pow(p->unit_length_cm, 3) / p->unit_time_s;
     const double M_mean = orc_RtoM(p, R);
     const double R_cubed = R * R * R;
+    /* :325-334 */
+    double Gamma_R_prefactor = 0.0;
+    if (recomb) {
+      Gamma_R_prefactor = (1.0 + redshift) * (1.0 + redshift) * R * (p->unit_length_cm / p->hubble_h) * ORC_SIGMA_HI *
+                          p->alpha_uv / (p->alpha_uv + 2.75) / 1.0e-12;
+      Gamma_R_prefactor *= (p->unit_mass_g / p->unit_time_s) * pow(p->unit_length_cm / p->hubble_h, -3.) *
+                           p->nion_phot_per_bary / ORC_PROTONMASS;
+    }
 
 #pragma omp parallel for schedule(static)
     for (long r = 0; r < (long)N * N; r++) {
       for (int z = 0; z < N; z++) {
         const long i_real = r * N + z;
         const long i_padded = r * PR + z;
-        double dv, sv, wv, xv = 0.0;
+        double dv, sv, wv, xv = 0.0, nv = 0.0;
         if (precision == 64) {
           dv = ((double*)fil[0])[i_padded];
           sv = ((double*)fil[1])[i_padded];
           wv = ((double*)fil[2])[i_padded];
-          if (p->include_spin_temp) xv = ((double*)fil[3])[i_padded];
+          if (p->include_spin_temp) xv = ((double*)fil[fx])[i_padded];
+          if (recomb) nv = ((double*)fil[fr])[i_padded];
         } else {
           dv = (double)((float*)fil[0])[i_padded];
           sv = (double)((float*)fil[1])[i_padded];
           wv = (double)((float*)fil[2])[i_padded];
-          if (p->include_spin_temp) xv = (double)((float*)fil[3])[i_padded];
+          if (p->include_spin_temp) xv = (double)((float*)fil[fx])[i_padded];
+          if (recomb) nv = (double)((float*)fil[fr])[i_padded];
         }
         const double density_over_mean = 1.0 + dv;
+        const double rec = recomb ? nv / density_over_mean : 0.0; /* :360-363 */
         const double f_coll_stars = sv / (M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * R_cubed / pixel_volume;
         const double weighted_sfr_density = wv / pixel_volume;
         const double neutral_fraction = p->include_spin_temp ? 1.0 - xv : 1.0;
@@ -385,6 +422,7 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
         if (f_coll_stars * eff > neutral_fraction * (1. + rec)) {
           if (xH[i_real] > ORC_REL_TOL) {
             if (uvb) J_21[i_real] = J_21_aux;
+            if (recomb) Gamma12[i_real] = (float)(Gamma_R_prefactor * weighted_sfr_density);
           }
           xH[i_real] = 0;
           r_bubble[i_real] = (float)R;
@@ -405,6 +443,11 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
   }
 
   /* find_HII_bubbles.c:451-494: sequential double sums, mass weight from the input deltax */
+  const mrx_rr_tables* rr = recomb ? orc_rr(p) : NULL;
+  if (recomb && !rr) return MRX_ENOMEM;
+  const float zstep = (float)(p->prev_redshift - redshift);
+  const float fabs_dtdz = (float)fabs(mrx_cosmo_dtdz(p->omega_lambda, p->omega_m, p->hubble_h, (float)redshift) /
+                                      p->hubble_h);
   double vol_xH = 0.0, mass_xH = 0.0, mass_weight = 0.0, vol_J21 = 0.0;
   for (long r = 0; r < (long)N * N; r++)
     for (int z = 0; z < N; z++) {
@@ -416,6 +459,11 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
       double density_over_mean = 1.0 + (double)deltax[i_padded];
       mass_xH += cell_xH * density_over_mean;
       mass_weight += density_over_mean;
+      if (recomb) { /* :472-477 */
+        const float z_eff = (float)((1. + redshift) * pow(density_over_mean, 1.0 / 3.0) - 1);
+        const double dNrec = mrx_rr_rate(rr, z_eff, Gamma12[i_real]) * fabs_dtdz * zstep * (1. - cell_xH);
+        N_rec[i_padded] += dNrec;
+      }
     }
   vol_xH /= total_n_cells;
   if (uvb) vol_J21 /= total_n_cells;

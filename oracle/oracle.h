@@ -95,6 +95,14 @@ int orc_brightness_temp_vel(const mrx_tb_vel_params* p, const float* xH, const f
 /* Its line-of-sight re-binning alone (:312-496): delta_T in place, vel converted. */
 int orc_rsd_los(const mrx_tb_vel_params* p, const float* xH, const float* vel, float* delta_T);
 
+/* _find_HII_bubbles with inhomogeneous recombinations (find_HII_bubbles.c:155-163,
+ * :246-302, :325-404, :472-477): N_rec (padded) and Gamma12 (unpadded) in/out;
+ * the recombination-rate table is recomb_tables.c (shared with the library). */
+int orc_find_hii_bubbles_rec(int precision, const mrx_hii_params* p, const float* deltax, const float* stars,
+                             const float* weighted_sfr, const float* x_e, float* N_rec, float* xH, float* r_bubble,
+                             float* J_21, float* z_at_ionization, float* J_21_at_ionization, float* Gamma12,
+                             mrx_hii_globals* out, float* max_ratio);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,7 @@ def lib():
         L.orc_construct_baryon_grids.argtypes = [C.POINTER(GridParams), C.c_long] + [vp] * 11
         L.orc_brightness_temp_vel.argtypes = [C.POINTER(TbVelParams)] + [vp] * 5 + [C.POINTER(C.c_double)]
         L.orc_rsd_los.argtypes = [C.POINTER(TbVelParams), vp, vp, vp]
+        L.orc_find_hii_bubbles_rec.argtypes = [i, C.POINTER(HiiParams)] + [vp] * 11 + [C.POINTER(HiiGlobals), vp]
         L.orc_set_threads.argtypes = [i]
         L.orc_get_threads.restype = i
         _lib = L
@@ -145,9 +146,16 @@ def filtered_field(N, box_size, filter_type, field_padded, R, apply_filter=1, cl
 
 
 def find_hii_bubbles(p, deltax, stars, wsfr, x_e=None, z_at_ion=None, J21_at_ion=None, precision=32,
-                     want_ratio=False):
-    """Runs the restated _find_HII_bubbles; returns a dict of unpadded outputs."""
+                     want_ratio=False, N_rec=None, Gamma12=None):
+    """Runs the restated _find_HII_bubbles; returns a dict of unpadded outputs.
+
+    With p.include_recombinations, N_rec (padded) and Gamma12 (unpadded) are
+    copied, updated and returned under the same names.
+    """
     N = p.grid_dim
+    if p.include_recombinations:
+        return _find_hii_bubbles_rec(p, deltax, stars, wsfr, x_e, z_at_ion, J21_at_ion, precision, want_ratio,
+                                     N_rec, Gamma12)
     shp = (N, N, N)
     xH = np.empty(shp, np.float32)
     rb = np.empty(shp, np.float32)
@@ -168,6 +176,54 @@ def find_hii_bubbles(p, deltax, stars, wsfr, x_e=None, z_at_ion=None, J21_at_ion
         "mass_weighted_global_xH": g.mass_weighted_global_xH,
         "volume_weighted_global_J_21": g.volume_weighted_global_J_21, "n_radii": g.n_radii,
     }
+
+
+def _find_hii_bubbles_rec(p, deltax, stars, wsfr, x_e, z_at_ion, J21_at_ion, precision, want_ratio, N_rec, Gamma12):
+    N = p.grid_dim
+    shp = (N, N, N)
+    xH = np.empty(shp, np.float32)
+    rb = np.empty(shp, np.float32)
+    J21 = np.empty(shp, np.float32) if p.uvb_flag else None
+    z_in = (np.full(shp, -1.0, np.float32) if z_at_ion is None else np.array(z_at_ion, np.float32, copy=True))
+    J21a = None
+    if p.uvb_flag:
+        J21a = np.zeros(shp, np.float32) if J21_at_ion is None else np.array(J21_at_ion, np.float32, copy=True)
+    nrec = np.array(N_rec, np.float32, copy=True)
+    g12 = np.zeros(shp, np.float32) if Gamma12 is None else np.array(Gamma12, np.float32, copy=True)
+    ratio = np.empty(shp, np.float32) if want_ratio else None
+    g = HiiGlobals()
+    args = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None for a in (deltax, stars, wsfr, x_e)]
+    rc = lib().orc_find_hii_bubbles_rec(precision, C.byref(p), *[fptr(a) for a in args], fptr(nrec), fptr(xH),
+                                        fptr(rb), fptr(J21), fptr(z_in), fptr(J21a), fptr(g12), C.byref(g),
+                                        fptr(ratio))
+    _chk(rc, "find_hii_bubbles_rec")
+    return {
+        "xH": xH, "r_bubble": rb, "J_21": J21, "z_at_ionization": z_in, "J_21_at_ionization": J21a,
+        "N_rec": nrec, "Gamma12": g12, "max_ratio": ratio,
+        "volume_weighted_global_xH": g.volume_weighted_global_xH,
+        "mass_weighted_global_xH": g.mass_weighted_global_xH,
+        "volume_weighted_global_J_21": g.volume_weighted_global_J_21, "n_radii": g.n_radii,
+    }
+
+
+def rr_tables_lib():
+    """The oracle library's recombination-table functions (recomb_tables.c), for the tests."""
+    L = lib()
+    d = C.c_double
+    for name, args in (("mrx_rr_No", [d, d, d, d]), ("mrx_rr_neutral_fraction", [d, d, d, d, C.c_int]),
+                       ("mrx_rr_Gamma_SS", [d, d, d, d]), ("mrx_cosmo_dtdz", [d, d, d, C.c_float])):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = d
+    for name, args in (("mrx_rr_A_MHR", [C.c_void_p, d]), ("mrx_rr_C_MHR", [C.c_void_p, d]),
+                       ("mrx_rr_beta_MHR", [C.c_void_p, d]), ("mrx_rr_rate", [C.c_void_p, C.c_float, d]),
+                       ("mrx_rr_recombination_rate", [C.c_void_p, d, d, d, C.c_int])):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = d
+    L.mrx_rr_init.argtypes = [C.c_void_p, d, d]
+    L.mrx_rr_init.restype = C.c_int
+    return L
 
 
 def brightness_temp(p, xH, deltax, TS_box=None):

@@ -1,0 +1,153 @@
+"""Oracle: inhomogeneous recombinations (recombinations.c, find_HII_bubbles.c:155-477).
+
+The host tables (recomb_tables.c, shared by the library and the oracle) are
+pinned against independent evaluations: the MHR00 normalisation A(z) against
+scipy's quad in ln D; the equilibrium neutral fraction against its quadratic
+and its small-chi limit; the self-shielding fit at Delta -> 0; the rate
+integral against scipy's quad of the same integrand; the splined table
+against the integral at its knots and the reference's index rounding and
+clamps; dt/dz against -1/((1+z) H(z)).  The bubble finder's recombination
+terms are pinned by closed forms: N_rec = 0 leaves xH unchanged, a uniform
+N_rec shifts the threshold to n_f (1 + N_rec), and the N_rec update of a
+uniform box is RR(z, Gamma12) |dt/dz| zstep (1 - xH).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, interpolate
+
+from meraxes_amd import params as P
+from oracle import oracle as O
+
+Y_HE = 0.24
+
+
+@pytest.fixture(scope="module")
+def rr():
+    L = O.rr_tables_lib()
+    No = L.mrx_rr_No(P.HUBBLE_H, P.OMEGA_M, P.BARYON_FRAC, Y_HE)
+    buf = C.create_string_buffer(2_000_000)
+    assert L.mrx_rr_init(buf, No, Y_HE) == 0
+    return L, buf, No
+
+
+def C_fn(z):
+    Cs = interpolate.CubicSpline(np.arange(12) + 2.0, [0.558, 0.599, 0.611, 0.769, 0.868, 0.930, 0.964, 0.983,
+                                                        0.993, 0.998, 0.999, 1.00], bc_type="natural")
+    return 1.0 if z >= 13 else (0.558 if z <= 2 else float(Cs(z)))
+
+
+def B_fn(z):
+    Bs = interpolate.CubicSpline(np.arange(5) + 2.0, [-2.23, -2.35, -2.48, -2.49, -2.50], bc_type="natural")
+    return -2.5 if z >= 6 else (-2.23 if z <= 2 else float(Bs(z)))
+
+
+def test_mhr_parameters_and_normalisation(rr):
+    L, t, _ = rr
+    for z in (2.0, 4.5, 8.0, 12.3, 20.0, 45.0):
+        assert L.mrx_rr_C_MHR(t, z) == pytest.approx(C_fn(z), abs=1e-12)
+        assert L.mrx_rr_beta_MHR(t, z) == pytest.approx(B_fn(z), abs=1e-12)
+        s = 2 * 7.61 / (3 * (1 + z))
+        f = (lambda u: math.exp(-(math.exp(-2 / 3 * u) - C_fn(z)) ** 2 / (2 * s * s)) * math.exp((B_fn(z) + 1) * u))
+        norm, _ = integrate.quad(f, math.log(1e-25), math.log(1e25), points=[0.0], limit=500, epsrel=1e-11)
+        # A is splined between integer z; at integers it is 1 / the integral (QAG to 1e-3)
+        if z == int(z):
+            assert L.mrx_rr_A_MHR(t, z) * norm == pytest.approx(1.0, rel=1e-4)
+
+
+def test_neutral_fraction_and_self_shielding(rr):
+    L, _, _ = rr
+    corr = 1.0 / (4.0 / Y_HE - 3)
+    aB = 2.59e-13
+    for n in (1e-6, 1e-4, 1e-2, 1.0):
+        for g in (1e-3, 0.1, 1.0, 30.0):
+            chi = L.mrx_rr_neutral_fraction(Y_HE, n, 1.0, g, 1)
+            a = (1 + corr) * n * aB / (g * 1e-12)
+            if a < 1e-5:
+                assert chi == pytest.approx(a, rel=1e-12)  # the small-chi branch returns chi itself
+            else:
+                # the smaller root of chi^2 + b chi + 1 = 0, which the reference forms as
+                # (-b - sqrt(b^2 - 4)) / 2 (cancelling for large |b|: relative error ~ eps b^2)
+                b = -2 - 1.0 / a
+                stable = 2.0 / (-b + math.sqrt(b * b - 4))
+                assert chi == pytest.approx(stable, rel=max(1e-12, 4e-16 * b * b))
+                assert 0 < chi < 1
+    # Rahmati et al. fit: no attenuation at vanishing density, strong at high density
+    assert L.mrx_rr_Gamma_SS(0.7, 1e-12, 1.0, 8.0) == pytest.approx(0.7, rel=1e-9)
+    assert L.mrx_rr_Gamma_SS(0.7, 1e4, 1.0, 8.0) < 0.05 * 0.7
+
+
+def test_rate_integral_and_table(rr):
+    L, t, No = rr
+    for z, g in ((6.0, 0.3), (8.0, 1.0), (12.4, 5.0)):
+        A, Cz, Bz = L.mrx_rr_A_MHR(t, z), L.mrx_rr_C_MHR(t, z), L.mrx_rr_beta_MHR(t, z)
+        nbar = No * (1 + z) ** 3
+
+        def f(u):
+            D = math.exp(u)
+            gam = L.mrx_rr_Gamma_SS(g, D, 1.0, z)
+            xe = 1.0 - L.mrx_rr_neutral_fraction(Y_HE, nbar * D, 1.0, gam, 1)
+            Pd = A * math.exp(-0.5 * ((D ** (-2 / 3) - Cz) / (2 * 7.61 / (3 * (1 + z)))) ** 2) * D ** Bz
+            return nbar * D * Pd * 2.59e-13 * xe * xe * D * D
+
+        ref, _ = integrate.quad(f, math.log(0.01), math.log(200), limit=200, epsrel=1e-10)
+        assert L.mrx_rr_recombination_rate(t, z, g, 1.0, 1) == pytest.approx(ref, rel=1e-2)
+    # the spline passes through its knots: row z_ct = 40 is z = 8, knot k is ln Gamma = -10 + k * 0.1f
+    for k in (0, 37, 100, 149):
+        lnG = -10.0 + float(np.float32(k) * np.float32(0.1))
+        g = float(np.float32(math.exp(lnG)))
+        direct = L.mrx_rr_recombination_rate(t, float(np.float32(40 * np.float32(0.2))), g, 1.0, 1)
+        assert L.mrx_rr_rate(t, 8.0, math.exp(lnG)) == pytest.approx(direct, rel=1e-12)
+    # nearest redshift row (z / 0.2 rounded), and the clamps
+    assert L.mrx_rr_rate(t, 8.09, 1.0) == L.mrx_rr_rate(t, 8.0, 1.0)
+    assert L.mrx_rr_rate(t, 8.11, 1.0) == L.mrx_rr_rate(t, 8.2, 1.0)
+    assert L.mrx_rr_rate(t, 8.0, math.exp(-10.5)) == 0.0
+    assert L.mrx_rr_rate(t, -3.0, 1.0) == L.mrx_rr_rate(t, 0.0, 1.0)
+
+
+def test_dtdz_closed_form():
+    L = O.rr_tables_lib()
+    for z in (0.0, 6.0, 8.4, 20.0):
+        z = float(np.float32(z))  # dtdz takes a float redshift
+        H = 3.2407789e-18 * P.HUBBLE_H * math.sqrt(P.OMEGA_M * (1 + z) ** 3 + P.OMEGA_L)
+        assert L.mrx_cosmo_dtdz(P.OMEGA_L, P.OMEGA_M, P.HUBBLE_H, z) == pytest.approx(-1 / ((1 + z) * H), rel=1e-9)
+
+
+def uniform_box(N, s_val, d_val=0.0):
+    d = np.zeros((N, N, N + 2), np.float32)
+    d[:, :, :N] = d_val
+    s = np.zeros_like(d)
+    s[:, :, :N] = s_val
+    return d, s, s.copy()
+
+
+def test_bubble_finder_recombination_terms(rr):
+    L, t, _ = rr
+    N, box, z, zprev = 16, 20.0, 8.0, 8.3
+    p = P.hii_params(N, box, z, include_recombinations=1, prev_redshift=zprev, r_bubble_max=5.0)
+    # f_coll * eps of a uniform box (R cancels for the top-hat RtoM): stars * eps / (Omega_m rho_c V_pix)
+    kf = p.efficiency / (p.omega_m * p.rho_crit * (box / N) ** 3)
+    s_ion = 1.5 / kf  # f_coll * eps = 1.5
+    d, s, w = uniform_box(N, s_ion)
+    zero = np.zeros_like(d)
+    r0 = O.find_hii_bubbles(p, d, s, w, N_rec=zero)
+    p_plain = P.hii_params(N, box, z, r_bubble_max=5.0)
+    rp = O.find_hii_bubbles(p_plain, d, s, w)
+    assert np.array_equal(r0["xH"], rp["xH"]) and (r0["xH"] == 0).all()
+    # every cell crossed at the first radius: Gamma12 = Gamma_R_prefactor(R0) * sfr / V
+    assert (r0["Gamma12"] > 0).all()
+    # a uniform N_rec = 0.6 moves the threshold to 1.6 > 1.5: nothing ionises, and
+    # the last step's partial xH (which the reference leaves without the 1 + rec) is 1 - 1.5 -> 0
+    nrec = zero.copy()
+    nrec[:, :, :N] = 0.6
+    r1 = O.find_hii_bubbles(p, d, s, w, N_rec=nrec)
+    assert (r1["r_bubble"] == 0).all() and (r1["xH"] == 0).all() and (r1["Gamma12"] == 0).all()
+    # N_rec update of the fully ionised uniform box: z_eff = z, dN = RR(z, Gamma12) |dt/dz| zstep
+    g12 = float(r0["Gamma12"][0, 0, 0])
+    dtdz = abs(L.mrx_cosmo_dtdz(p.omega_lambda, p.omega_m, p.hubble_h, z) / p.hubble_h)
+    dN = L.mrx_rr_rate(t, z, g12) * float(np.float32(dtdz)) * float(np.float32(zprev - z))
+    assert dN > 0
+    np.testing.assert_array_equal(r0["N_rec"][:, :, :N], np.float32(dN))
+    assert (r0["N_rec"][:, :, N:] == 0).all()
