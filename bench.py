@@ -19,11 +19,12 @@ oracle (the C restatement of the reference's FFTW-float path) on the host cores
 over a bounded sample of the same workload.  Unless --no-extras is given the
 snapshot run also times the hand-written adjoint against its forward at 256^3
 and 512^3, the batched bubble finder (96 uneven boxes, one graph launch)
-against the per-box loop, the galaxy gridding (2^24 galaxies onto 512^3) and
-delta_T with peculiar velocities and line-of-sight RSD at 512^3.
+against the per-box loop, the galaxy gridding (2^24 galaxies onto 512^3),
+delta_T with peculiar velocities and line-of-sight RSD at 512^3, and the HDF5
+grid files of one snapshot written from HBM at 256^3.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512]
-           [--workload snapshot|bubbles|batched] [--no-extras] [--no-cpu-baseline]
+           [--workload snapshot|bubbles|batched|gridding|grid_files] [--no-extras] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -345,6 +346,81 @@ def bench_tb_vel(ctx, N=512, reps=3):
             "config": f"{N}^3, {L:g} Mpc/h, z=8, spin temperature, RSD with 50 sub-cells, synthetic"}
 
 
+def bench_grid_files(ctx, N=256, reps=2):
+    """The HDF5 grid files (save_reion_input_grids + save_reion_output_grids) from HBM.
+
+    Every flag on: 4 input grids (3 padded and converted, sfr and weighted_sfr
+    to M_sun/yr) and 10 output grids (x_e_box de-padded), N^3 float32 each,
+    written to a temporary directory on the local disk and deleted after each
+    repetition.  Reported against two ceilings measured in the same run: a
+    device-to-pinned-host copy of one grid, and a plain write() of the same
+    bytes to a file in the same directory (no fsync, as HDF5 does not sync).
+    """
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from meraxes_amd import abi, reion
+    from meraxes_amd import params as P
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    pad = lambda: torch.rand((N, N, N + 2), generator=g, device=dev)  # noqa: E731
+    real = lambda: torch.rand((N, N, N), generator=g, device=dev)  # noqa: E731
+    inputs = [pad() for _ in range(4)]
+    grids = {n: real() for n in ("xH", "z_at_ionization", "r_bubble", "J_21", "J_21_at_ionization", "Mvir_crit",
+                                 "TS_box", "Tk_box", "delta_T")}
+    grids["x_e_box_prev"] = pad()
+    avg = {n: 0.5 for n in abi.OUTPUT_AVERAGES}
+    p = P.save_params(N, uvb_flag=1, include_spin_temp=1, compute_21cm_bright_temp=1)
+    d = tempfile.mkdtemp(prefix="mrx_grids_", dir=os.environ.get("TMPDIR", "/tmp"))
+    path = os.path.join(d, "meraxes_grids_24.hdf5")
+    nbytes = 14 * 4 * N**3
+    try:
+        def once():
+            reion.save_reion_input_grids(ctx, p, path, *inputs)
+            reion.save_reion_output_grids(ctx, p, path, grids, avg)
+
+        torch.cuda.synchronize()
+        once()
+        os.remove(path)
+        t = 0.0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            once()
+            t += time.perf_counter() - t0
+            os.remove(path)
+        dt = t / reps
+        # ceilings: one grid device -> pinned host, and the same bytes through write()
+        host = torch.empty((N, N, N), dtype=torch.float32, pin_memory=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(4):
+            host.copy_(grids["xH"])
+        torch.cuda.synchronize()
+        d2h = 4 * 4 * N**3 / (time.perf_counter() - t0)
+        arr = host.numpy()
+        raw = os.path.join(d, "raw.bin")
+        t0 = time.perf_counter()
+        with open(raw, "wb") as f:
+            for _ in range(14):
+                arr.tofile(f)
+        wr = nbytes / (time.perf_counter() - t0)
+        os.remove(raw)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"[bench] grid files at {N}^3: {dt * 1e3:.1f} ms for {nbytes / 2**20:.0f} MiB "
+        f"({nbytes / dt / 1e9:.2f} GB/s; D2H ceiling {d2h / 1e9:.1f} GB/s, write() ceiling {wr / 1e9:.2f} GB/s)")
+    del inputs, grids
+    torch.cuda.empty_cache()
+    return {"ms_per_snapshot": dt * 1e3, "GB_s": nbytes / dt / 1e9, "bytes": nbytes, "grid": N,
+            "d2h_ceiling_GB_s": d2h / 1e9, "write_ceiling_GB_s": wr / 1e9,
+            "frac_of_write_ceiling": (nbytes / dt) / wr,
+            "config": f"{N}^3, all flags: 4 input + 10 output grids from HBM to one HDF5 file, synthetic"}
+
+
 BATCH_PER_SIZE = 32
 
 
@@ -451,7 +527,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["snapshot", "bubbles", "batched", "gridding"], default="snapshot")
+    ap.add_argument("--workload", choices=["snapshot", "bubbles", "batched", "gridding", "grid_files"],
+                    default="snapshot")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the adjoint, batched and gridding parts of the snapshot run")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -498,6 +575,18 @@ def main():
                "warmup": 1, "ms_per_step": b["ms_per_call"], "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": {"workload": b["config"]},
                "gridding": b}
+        print(json.dumps(out), flush=True)
+        return
+
+    if args.workload == "grid_files":
+        ctx = Context(0)
+        b = bench_grid_files(ctx, N=args.grid, reps=max(args.steps, 1))
+        ctx.close()
+        out = {"metric": "bytes/s of one snapshot's HDF5 grid files written from HBM", "value": b["GB_s"] * 1e9,
+               "unit": "bytes/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+               "ms_per_step": b["ms_per_snapshot"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": {"workload": b["config"]},
+               "grid_files": b}
         print(json.dumps(out), flush=True)
         return
 
@@ -640,6 +729,7 @@ def main():
         out["batched"] = bench_batched(ctx, 2, 1)
         out["gridding"] = bench_gridding(ctx)
         out["tb_velocities"] = bench_tb_vel(ctx)
+        out["grid_files"] = bench_grid_files(ctx)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(snapshot)

@@ -31,6 +31,7 @@ extern "C" {
 #define MRX_EDEVICE (-3)      /* HIP runtime error */
 #define MRX_EUNSUPPORTED (-4) /* grid size or option outside what this build implements */
 #define MRX_EFILTER (-5)      /* invalid filter type (reionization.c:2160-2165 aborts there) */
+#define MRX_EIO (-6)          /* a grid file could not be created, opened or written (mrx_io.h) */
 
 /* Filter windows of filter() (reionization.c:2110-2170). */
 #define MRX_FILTER_TOPHAT 0 /* real-space top-hat 3(sin x - x cos x)/x^3, x = kR, applied when x > 1e-4 */

@@ -6,11 +6,12 @@ entry point raises.  Build it with ``python -m meraxes_amd.build``.
 import ctypes as C
 import os
 
-from .abi import (ERRORS, GridParams, GridStats, HiiBatchBox, HiiGlobals, HiiParams, TbParams, TbVelParams,
-                  TsGlobals, TsParams)
+from .abi import (ERRORS, GridParams, GridStats, HiiBatchBox, HiiGlobals, HiiParams, OutputGrids, SaveParams,
+                  TbParams, TbVelParams, TsGlobals, TsParams)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmrx_hip.so")
+IO_LIB_PATH = os.path.join(HERE, "libmrx_io.so")
 
 # every symbol include/mrx_reion.h declares
 EXPORTS = (
@@ -22,7 +23,11 @@ EXPORTS = (
     "mrx_brightness_temp_vel", "mrx_find_hii_bubbles_rec",
 )
 
+# every symbol include/mrx_io.h declares (libmrx_io.so)
+IO_EXPORTS = ("mrx_grids_fname", "mrx_save_reion_input_grids", "mrx_save_reion_output_grids", "mrx_io_abi_sizeof")
+
 _lib = None
+_io = None
 
 
 class MrxError(RuntimeError):
@@ -71,6 +76,25 @@ def load():
     L.mrx_abi_sizeof.argtypes = [i]
     L.mrx_abi_sizeof.restype = C.c_size_t
     _lib = L
+    return L
+
+
+def load_io():
+    """The grid-file library libmrx_io.so (over HDF5); raises OSError if it is not built."""
+    global _io
+    if _io is not None:
+        return _io
+    load()  # the context library it links
+    if not os.path.exists(IO_LIB_PATH):
+        raise OSError(f"{IO_LIB_PATH} is not built (python -m meraxes_amd.build)")
+    L = C.CDLL(IO_LIB_PATH)
+    vp, i = C.c_void_p, C.c_int
+    L.mrx_grids_fname.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, i, i]
+    L.mrx_save_reion_input_grids.argtypes = [vp, C.POINTER(SaveParams), C.c_char_p] + [vp] * 4
+    L.mrx_save_reion_output_grids.argtypes = [vp, C.POINTER(SaveParams), C.c_char_p, C.POINTER(OutputGrids)]
+    L.mrx_io_abi_sizeof.argtypes = [i]
+    L.mrx_io_abi_sizeof.restype = C.c_size_t
+    _io = L
     return L
 
 

@@ -12,6 +12,7 @@ MRX_ENOMEM = -2
 MRX_EDEVICE = -3
 MRX_EUNSUPPORTED = -4
 MRX_EFILTER = -5
+MRX_EIO = -6
 
 ERRORS = {
     MRX_EINVAL: "MRX_EINVAL",
@@ -19,6 +20,7 @@ ERRORS = {
     MRX_EDEVICE: "MRX_EDEVICE",
     MRX_EUNSUPPORTED: "MRX_EUNSUPPORTED",
     MRX_EFILTER: "MRX_EFILTER",
+    MRX_EIO: "MRX_EIO",
 }
 
 FILTER_TOPHAT = 0
@@ -220,3 +222,33 @@ class TbVelParams(C.Structure):
         ("omega_lambda", C.c_double),
         ("omega_r", C.c_double),
     ]
+
+
+class SaveParams(C.Structure):
+    """mrx_save_params (include/mrx_io.h): flags and units of the grid files."""
+
+    _fields_ = [
+        ("grid_dim", C.c_int),
+        ("uvb_flag", C.c_int),
+        ("include_spin_temp", C.c_int),
+        ("compute_21cm_bright_temp", C.c_int),
+        ("compute_ps", C.c_int),
+        ("ps_length", C.c_int),
+        ("pointers_on_device", C.c_int),
+        ("pad_", C.c_int),
+        ("unit_mass_g", C.c_double),
+        ("unit_time_s", C.c_double),
+    ]
+
+
+OUTPUT_GRID_NAMES = ("xH", "z_at_ionization", "r_bubble", "J_21", "J_21_at_ionization", "Mvir_crit", "TS_box",
+                     "Tk_box", "x_e_box_prev", "delta_T", "PS_k", "PS_data", "PS_error")
+OUTPUT_AVERAGES = ("volume_weighted_global_xH", "mass_weighted_global_xH", "volume_weighted_global_J_21",
+                   "volume_ave_TS", "volume_ave_TK", "volume_ave_xe", "volume_ave_J_alpha", "volume_ave_xalpha",
+                   "volume_ave_Xheat", "volume_ave_Xion", "volume_ave_Tb")
+
+
+class OutputGrids(C.Structure):
+    """mrx_output_grids (include/mrx_io.h): the output grids and averages of one snapshot."""
+
+    _fields_ = [(n, C.c_void_p) for n in OUTPUT_GRID_NAMES] + [(n, C.c_double) for n in OUTPUT_AVERAGES]

@@ -13,6 +13,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmrx_hip.so")
+IO_OUT = os.path.join(HERE, "libmrx_io.so")
+# the HDF5 C library (serial, with the high-level H5LT routines) of this image
+HDF5_ROOT = os.environ.get("MRX_HDF5_ROOT", "/opt/conda")
 ARCH = os.environ.get("MRX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -59,7 +62,34 @@ def build(verbose=False):
     if verbose and err:
         print(err, file=sys.stderr)
     os.replace(tmp, OUT)
+    build_io(verbose)
     return OUT
+
+
+def build_io(verbose=False):
+    """libmrx_io.so: the grid files (include/mrx_io.h) over the HDF5 C library.
+
+    A library of its own so that libmrx_hip.so does not depend on HDF5; it
+    links libmrx_hip.so (the context) from its own directory.
+    """
+    tmp = IO_OUT + ".tmp"
+    cmd = ([hipcc()] + flags() + ["-idirafter", os.path.join(HDF5_ROOT, "include"), "-o", tmp,
+                                  os.path.join(CSRC, "io.hip"), "-L" + HERE, "-lmrx_hip",
+                                  "-L" + os.path.join(HDF5_ROOT, "lib"), "-lhdf5_hl", "-lhdf5",
+                                  # the system's own libraries first: the HDF5 directory also holds
+                                  # an older libstdc++ and libgcc_s
+                                  "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/usr/lib/x86_64-linux-gnu",
+                                  "-Wl,-rpath," + os.path.join(HDF5_ROOT, "lib")])
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed (libmrx_io.so):\n{err}")
+    if verbose and err:
+        print(err, file=sys.stderr)
+    os.replace(tmp, IO_OUT)
+    return IO_OUT
 
 
 if __name__ == "__main__":

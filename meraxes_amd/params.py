@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .abi import GridParams, HiiParams, TbParams, TbVelParams, TsParams
+from .abi import GridParams, HiiParams, SaveParams, TbParams, TbVelParams, TsParams
 
 # src/meraxes.h:21-44
 GRAVITY = 6.672e-8
@@ -122,6 +122,22 @@ def tb_vel_params(grid_dim, box_size, redshift, pec_vels=3, trees_id=1, on_devic
     p.box_size = box_size
     p.omega_lambda = OMEGA_L
     p.omega_r = 0.0
+    return p
+
+
+def save_params(grid_dim, uvb_flag=1, include_spin_temp=1, compute_21cm_bright_temp=1, compute_ps=0, ps_length=0,
+                on_device=0):
+    """mrx_save_params: flags of the grid files, Tiamat units (UnitTime = UnitLength / UnitVelocity)."""
+    p = SaveParams()
+    p.grid_dim = grid_dim
+    p.uvb_flag = uvb_flag
+    p.include_spin_temp = include_spin_temp
+    p.compute_21cm_bright_temp = compute_21cm_bright_temp
+    p.compute_ps = compute_ps
+    p.ps_length = ps_length
+    p.pointers_on_device = on_device
+    p.unit_mass_g = UNIT_MASS_G
+    p.unit_time_s = units()["unit_time_s"]
     return p
 
 

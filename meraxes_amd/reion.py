@@ -9,6 +9,10 @@ Same names and argument meaning as the reference functions each call replaces:
 * ``brightness_temp``   -> ComputeBrightnessTemperatureBox (BrightnessTemperature.c:24-549)
 * ``compute_ts``        -> _ComputeTs (ComputeTs.c:78-1020)
 * ``power_spectrum``    -> Compute_PS (ComputePowerSpectrum.c:44-255)
+* ``construct_baryon_grids`` -> construct_baryon_grids (reionization.c:1353-1615)
+* ``brightness_temp_vel`` -> the peculiar-velocity branch (BrightnessTemperature.c:126-510)
+* ``gen_grids_fname``, ``save_reion_input_grids``, ``save_reion_output_grids``
+  -> the HDF5 grid files (reionization.c:1640-2050), through libmrx_io.so
 * ``HiiBatch``          -> many boxes of _find_HII_bubbles in one graph launch
   (no counterpart in the reference, which runs one box per call)
 * ``power_spectrum_adjoint``, ``snapshot_adjoint`` -> backward passes of
@@ -22,9 +26,11 @@ spectrum (N, N, N/2+1) complex64.  Invalid arguments raise MrxError with the
 library's message, where the reference aborts (e.g. reionization.c:2160-2165).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
+from . import _lib, abi
 from ._lib import Context, MrxError, load  # noqa: F401
 from .abi import GridStats, HiiBatchBox, HiiGlobals, TsGlobals, TsParams
 
@@ -219,6 +225,64 @@ def brightness_temp_vel(ctx, params, xH, deltax, TS_box, vel, out=None):
     ctx.check(ctx.lib.mrx_brightness_temp_vel(ctx.ptr, C.byref(params), _ptr(xH), _ptr(deltax), _ptr(TS_box),
                                               _ptr(vel), _ptr(out), C.byref(ave)), "brightness_temp_vel")
     return out, ave.value
+
+
+def gen_grids_fname(output_dir, file_name_galaxies, snapshot, relative=False):
+    """gen_grids_fname (reionization.c:1640-1646): the snapshot's grid file name."""
+    L = _lib.load_io()
+    buf = C.create_string_buffer(4096)
+    n = L.mrx_grids_fname(buf, len(buf), (output_dir or "").encode(), file_name_galaxies.encode(), int(snapshot),
+                          1 if relative else 0)
+    if n < 0:
+        raise _lib.MrxError(n, "grids_fname: name does not fit")
+    return buf.value.decode()
+
+
+def save_reion_input_grids(ctx, params, path, deltax, stars, weighted_sfr, sfr=None):
+    """save_reion_input_grids (reionization.c:1648-1754): creates the grid file at ``path``.
+
+    ``params`` is an mrx_save_params; the grids are padded, on the device
+    (torch) or the host (numpy).  sfr is needed with include_spin_temp.
+    """
+    L = _lib.load_io()
+    on_dev = _is_dev(deltax)
+    params.pointers_on_device = 1 if on_dev else 0
+    _bind_stream(ctx, on_dev)
+    ctx.check(L.mrx_save_reion_input_grids(ctx.ptr, C.byref(params), os.fsencode(path), _ptr(deltax), _ptr(stars),
+                                           _ptr(weighted_sfr), _ptr(sfr)), "save_reion_input_grids")
+
+
+def save_reion_output_grids(ctx, params, path, grids, averages=None):
+    """save_reion_output_grids (reionization.c:1786-2050): adds the output grids to ``path``.
+
+    ``grids`` maps the names of mrx_output_grids (xH, z_at_ionization,
+    r_bubble, J_21, J_21_at_ionization, Mvir_crit, TS_box, Tk_box,
+    x_e_box_prev (padded), delta_T; PS_k, PS_data, PS_error as host float32
+    arrays) to arrays; ``averages`` maps the attribute names
+    (volume_weighted_global_xH, ...) to floats.
+    """
+    L = _lib.load_io()
+    g = abi.OutputGrids()
+    keep = []
+    on_dev = _is_dev(grids["xH"])
+    for name in abi.OUTPUT_GRID_NAMES:
+        a = grids.get(name)
+        if a is None:
+            continue
+        if name.startswith("PS_"):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+        elif _is_dev(a) != on_dev:
+            raise ValueError(f"{name}: every grid must be on the same side (host or device)")
+        keep.append(a)
+        setattr(g, name, _ptr(a))
+    for name, v in (averages or {}).items():
+        if name not in abi.OUTPUT_AVERAGES:
+            raise KeyError(name)
+        setattr(g, name, float(v))
+    params.pointers_on_device = 1 if on_dev else 0
+    _bind_stream(ctx, on_dev)
+    ctx.check(L.mrx_save_reion_output_grids(ctx.ptr, C.byref(params), os.fsencode(path), C.byref(g)),
+              "save_reion_output_grids")
 
 
 def profile(ctx, enable=True):
