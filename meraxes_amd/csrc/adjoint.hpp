@@ -95,28 +95,18 @@ __device__ __forceinline__ D7 ts_cell_dual(const TsDev& t, const double* tabs, f
                                            const float* shells, long ncell, long i, D7* xn_out)
 {
   const XeBracket br = xe_bracket(t, xe0);
-  const int ns = t.nsteps, per = kNXHII * ns;
   double H = 0, I = 0, Ly = 0, St = 0, dH = 0, dI = 0, dLy = 0;
   if (!t.no_light) {
-    for (int ct = 0; ct < t.nsteps; ++ct) {
-      const float sv = shells[(long)ct * ncell + i];
-      const double SFR = (double)sv * t.sfr_factor;
-      double dfh, dfi, dfl;
-      const int at = br.lo * ns + ct;
-      const double fh = interp(tabs[at], tabs[3 * per + at], br, &dfh);
-      const double fi = interp(tabs[per + at], tabs[4 * per + at], br, &dfi);
-      const double fl = interp(tabs[2 * per + at], tabs[5 * per + at], br, &dfl);
-      const double dt_dzpp = t.dt_dzpp[ct], dzpp = t.dzpp[ct];
-      const double zpp_integrand = SFR * t.zpp_pow[ct];
-      H += dt_dzpp * dzpp * zpp_integrand * fh;
-      I += dt_dzpp * dzpp * zpp_integrand * fi;
-      Ly += dt_dzpp * dzpp * zpp_integrand * fl;
-      St += SFR * t.P2 * t.zpp1[ct] * t.sum_lyn[ct] * dt_dzpp * dzpp;
-      const double w = dt_dzpp * dzpp * zpp_integrand;
-      dH += w * dfh;
-      dI += w * dfi;
-      dLy += w * dfl;
-    }
+    // the forward's sums (ts_shell_sums), so the primal values are its own;
+    // the slope parts carry d/d(x_e) through fr
+    const ShellSums S = ts_shell_sums(t, tabs, br.lo, shells, ncell, i);
+    H = S.hf + br.fr * S.hs;
+    I = S.xf + br.fr * S.xs;
+    Ly = S.lf + br.fr * S.ls;
+    St = S.st;
+    dH = S.hs * br.dfr;
+    dI = S.xs * br.dfr;
+    dLy = S.ls * br.dfr;
   }
   const D7 dlD = D7::var((double)dl, 0);
   const D7 xD = D7::var((double)xe0, 1);
@@ -166,9 +156,9 @@ __global__ __launch_bounds__(kNT) void k_snapshot_adjoint_cell(
   const float* __restrict__ gdT, float* __restrict__ g_delta, float* __restrict__ g_stars,
   float* __restrict__ g_xe_prev, float* __restrict__ g_Tk, double* __restrict__ gS, AdjCellArgs a)
 {
-  extern __shared__ double tabs[];  // the x_e tables when ts_mode == 2 (ts_cell_lds_bytes)
+  extern __shared__ double tabs[];  // the folded shell coefficients when ts_mode == 2 (ts_cell_lds_bytes)
   const TsDev& t = *tg;
-  if (a.ts_mode == 2) load_ts_tables(t, tabs);
+  if (a.ts_mode == 2) load_ts_coef(t, tabs);
   __syncthreads();
   const int N = a.N;
   const long ncell = (long)N * N * N;

@@ -239,8 +239,8 @@ struct BatchPipe {
           c.T, nc, (long)c.F * nc, c.tw, c.boxes_dev, c.state, c.Rtab, c.J21c, c.S, c.partials, a);
       MRX_TRY(check_launch(ctx, "k_c2r_threshold_batch"));
     }
-    k_reduce_partials<<<c.nbox, kNT, 0, c.stream>>>(c.partials, nb, 4, c.sums);
-    return check_launch(ctx, "k_reduce_partials (batch)");
+    // the same reduction as the single-box path, so each box's globals are identical to it
+    return reduce_partials(ctx, c.stream, c.partials, nb, 4, c.nbox, c.partials + (size_t)nb * 4 * c.nbox, c.sums);
   }
 
   // Allocates the cohort's buffers and uploads its tables.
@@ -261,7 +261,8 @@ struct BatchPipe {
     MRX_TRY(alloc((void**)&c.tw, sizeof(float2) * N));
     MRX_TRY(alloc((void**)&c.Rtab, sizeof(float) * c.S));
     MRX_TRY(alloc((void**)&c.wtab, sizeof(float) * (size_t)(NH + 1) * (NH + 1) * (NH + 1)));
-    MRX_TRY(alloc((void**)&c.partials, sizeof(double) * (size_t)nb_threshold() * 4 * c.nbox));
+    MRX_TRY(alloc((void**)&c.partials,
+                  sizeof(double) * ((size_t)nb_threshold() * 4 + (size_t)kRedBlocks * kRedMaxV) * c.nbox));
     MRX_TRY(alloc((void**)&c.sums, sizeof(double) * 4 * c.nbox));
     MRX_TRY(alloc((void**)&c.boxes_dev, sizeof(BoxDev) * c.nbox));
     MRX_TRY(alloc((void**)&c.J21c, sizeof(double) * c.nbox * c.S));

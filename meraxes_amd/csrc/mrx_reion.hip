@@ -158,6 +158,44 @@ int check_launch(mrx_ctx* ctx, const char* what)
   return MRX_OK;
 }
 
+// Sums the nb x nv row-major partials of each of nbox boxes (boxes
+// contiguous) into out[box * nv + v]: in one workgroup per box for short
+// arrays, otherwise in two levels (k_reduce_rows, then one workgroup per box)
+// through `scratch`, kRedBlocks * kRedMaxV * nbox doubles.  The order of the
+// additions depends only on (nb, nv), never on timing.
+int reduce_partials(mrx_ctx* ctx, hipStream_t st, const double* partials, int nb, int nv, int nbox, double* scratch,
+                    double* out)
+{
+  if (nv > kRedMaxV || nb < 32 * kNT || !scratch) {
+    k_reduce_partials<<<nbox, kNT, 0, st>>>(partials, nb, nv, out);
+    return check_launch(ctx, "k_reduce_partials");
+  }
+  const int G = std::min(kRedBlocks, (nb + kNT - 1) / kNT);
+  const int chunk = (nb + G - 1) / G;
+  k_reduce_rows<<<dim3(G, nbox), kNT, 0, st>>>(partials, nb, nv, chunk, scratch);
+  MRX_TRY(check_launch(ctx, "k_reduce_rows"));
+  k_reduce_partials<<<nbox, kNT, 0, st>>>(scratch, G, nv, out);
+  return check_launch(ctx, "k_reduce_partials");
+}
+
+// Workgroups of kNT threads of `kernel` (with `lds` bytes of dynamic LDS) that
+// are resident on the context's device at once: the grid of a grid-stride
+// kernel that fills the device in one wave.
+template <class K>
+int resident_blocks(mrx_ctx* ctx, K kernel, size_t lds)
+{
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNT, lds) != hipSuccess || per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus < 1) {
+    (void)hipGetLastError();
+    cus = 256;
+  }
+  return per_cu * cus;
+}
+
 int finish(mrx_ctx* ctx)
 {
   if (!ctx->user_stream) HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -568,7 +606,7 @@ struct HiiRun {
       const int S = ladder(p, R);
       if (S < 1) return set_err(ctx, MRX_EINVAL, "radius ladder is empty");
       if (S > kMaxSteps) return set_err(ctx, MRX_EUNSUPPORTED, "radius ladder has %d steps (max %d)", S, kMaxSteps);
-      MRX_TRY(ensure_partials(ctx, (size_t)P::nblocks_threshold() * 4 + 8));
+      MRX_TRY(ensure_partials(ctx, (size_t)P::nblocks_threshold() * 4 + 8 + kRedBlocks * kRedMaxV));
       if (!ctx->Rtab) HIP_TRY(hipMalloc(&ctx->Rtab, sizeof(float) * 256));
       if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
       std::vector<float> Rf(S);
@@ -636,10 +674,10 @@ struct HiiRun {
       }
       const int nb = P::nblocks_threshold();
       {
-      PROF("reduce_partials");
-      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nb, 4, ctx->sums);
+        PROF("reduce_partials");
+        MRX_TRY(reduce_partials(ctx, ctx->stream, ctx->partials, nb, 4, 1, ctx->partials + (size_t)nb * 4 + 8,
+                                ctx->sums));
       }
-      MRX_TRY(check_launch(ctx, "k_reduce_partials"));
       double h[4];
       HIP_TRY(hipMemcpyAsync(h, ctx->sums, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
       HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -746,6 +784,20 @@ void fill_ts_dev(TsDev& t, const mrx_ts_data* d, const mrx_ts_snap* s, const mrx
   }
   for (int x = 0; x < kNXHII; ++x) t.XH[x] = d->xi_XHII[x];
   const float zp = (float)s->zp;
+  // the folded shell coefficients of k_ts_cell (see TsDev::cw)
+  const double P2 = pow(1 + zp, 2);
+  for (int c = 0; c < s->nsteps; ++c) {
+    const double w = t.sfr_factor * t.dt_dzpp[c] * t.dzpp[c] * t.zpp_pow[c];
+    for (int x = 0; x < kNXHII; ++x) {
+      t.cw[x][c][0] = w * t.fheat[x][c];
+      t.cw[x][c][1] = w * t.fion[x][c];
+      t.cw[x][c][2] = w * t.flya[x][c];
+      t.cw[x][c][3] = w * t.sheat[x][c];
+      t.cw[x][c][4] = w * t.sion[x][c];
+      t.cw[x][c][5] = w * t.slya[x][c];
+    }
+    t.cst[c] = t.sfr_factor * P2 * t.zpp1[c] * t.sum_lyn[c] * t.dt_dzpp[c] * t.dzpp[c];
+  }
   t.zp_f = zp;
   t.dzp = s->dzp;
   t.dt_dzp = s->dt_dzp;
@@ -814,8 +866,11 @@ struct TsRun {
       const double box_size = p->box_size / h;
       const double pixel_volume = pow(box_size / (double)N, 3);
       const int nbc = (int)((ncell + kNT - 1) / kNT);
-      int nred = nbc;  // workgroups whose 7 partial sums are reduced at the end
-      MRX_TRY(ensure_partials(ctx, (size_t)nbc * 7 + 2 * 1024 + 16));
+      // workgroups of the cell kernels, whose 7 partial sums are reduced at the end
+      // (one cell per thread for k_ts_cell; the homogeneous kernel is grid-stride)
+      int nred = nbc;
+      const int nred_h = (int)std::min<long>(nbc, resident_blocks(ctx, k_ts_homogeneous, 0));
+      MRX_TRY(ensure_partials(ctx, (size_t)nbc * 7 + 2 * 1024 + 16 + kRedBlocks * kRedMaxV));
       if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
       if (!ctx->ts_dev) HIP_TRY(hipMalloc(&ctx->ts_dev, sizeof(TsDev)));
       double collapse_fraction = 0.0, x_e_ave = 0.0;
@@ -852,9 +907,10 @@ struct TsRun {
         if (prepare_only) return MRX_OK;
         {
           PROF("homogeneous");
-          k_ts_homogeneous<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, deltax, xe_prev, Tk, TS, N, xe0, T0,
-                                                         ctx->partials);
+          k_ts_homogeneous<<<nred_h, kNT, 0, ctx->stream>>>(ctx->ts_dev, deltax, xe_prev, Tk, TS, N, xe0, T0,
+                                                           ctx->partials);
         }
+        nred = nred_h;
         MRX_TRY(check_launch(ctx, "k_ts_homogeneous"));
         MRX_TRY(fcoll(kLFactor * box_size / (float)N, 0));
         x_e_ave = 0.0;
@@ -904,8 +960,8 @@ struct TsRun {
         }
         MRX_TRY(check_launch(ctx, "k_ts_cell"));
       }
-      k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(ctx->partials, nred, 7, ctx->sums);
-      MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+      MRX_TRY(reduce_partials(ctx, ctx->stream, ctx->partials, nred, 7, 1,
+                              ctx->partials + (size_t)nbc * 7 + 2 * 1024 + 16, ctx->sums));
       double hs[7];
       HIP_TRY(hipMemcpyAsync(hs, ctx->sums, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
       HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -639,6 +639,38 @@ __global__ __launch_bounds__(kNT) void k_reduce_partials(const double* __restric
   }
 }
 
+// First level of a large reduction: workgroup (b, box) sums rows
+// [b chunk, (b + 1) chunk) of the box's nb x nv row-major partials (boxes
+// contiguous) into row b of the box's block of `out` (gridDim.x rows of nv).
+// Every lane reads whole rows; nv <= kRedMaxV.  The partition depends only on
+// (nb, gridDim.x), so the sums are the same on every run.
+constexpr int kRedMaxV = 8;
+constexpr int kRedBlocks = 128;
+__global__ __launch_bounds__(kNT) void k_reduce_rows(const double* __restrict__ partials, int nb, int nv, int chunk,
+                                                   double* __restrict__ out)
+{
+  __shared__ double red[kNT];
+  partials += (long)blockIdx.y * nb * nv;
+  out += ((long)blockIdx.y * gridDim.x + blockIdx.x) * nv;
+  const int r0 = blockIdx.x * chunk;
+  const int r1 = min(nb, r0 + chunk);
+  double acc[kRedMaxV];
+#pragma unroll
+  for (int v = 0; v < kRedMaxV; ++v) acc[v] = 0.0;
+  for (int r = r0 + (int)threadIdx.x; r < r1; r += kNT) {
+    const double* row = partials + (long)r * nv;
+#pragma unroll
+    for (int v = 0; v < kRedMaxV; ++v)
+      if (v < nv) acc[v] += row[v];
+  }
+#pragma unroll
+  for (int v = 0; v < kRedMaxV; ++v) {
+    if (v >= nv) break;
+    const double t = block_sum<kNT>(acc[v], red);
+    if (threadIdx.x == 0) out[v] = t;
+  }
+}
+
 // ------------------------------------------- fused C2R + bubble threshold
 struct StepArgs {
   double M_mean;        // RtoM(R) (find_HII_bubbles.c:339)

@@ -63,6 +63,14 @@ struct TsDev {
   double No, N_b0, f_H, f_He, hubble_h;
   float zp_f;
   KappaSpline kHH, keH, kpH;
+  // the shell sums folded per (x_e bracket, shell): w_c = sfr_factor dt_dzpp dzpp zpp_pow
+  // times {fheat, fion, flya, sheat, sion, slya}[lo][c], so that with the float
+  // shell value v_c the cell's heat sum is sum_c v_c cw[lo][c][0] + fr sum_c v_c cw[lo][c][3]
+  // (the same terms as the reference's per-shell (slope fr + f) w SFR, summed in
+  // another order); cst[c] = sfr_factor P2 (1 + zpp) sum_lyn dt_dzpp dzpp for the
+  // stellar Ly-alpha sum
+  double cw[kNXHII][kTsMaxSteps][6];
+  double cst[kTsMaxSteps];
 };
 
 #pragma clang fp contract(off)
@@ -199,22 +207,65 @@ __device__ __forceinline__ double dev_alpha_A(double T) { return alpha_A_t<doubl
 
 // Per-cell spin-temperature update (ComputeTs.c:743-900); 7 block partial sums:
 // J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e.
-// Dynamic LDS: the six x_e-interpolation tables (values, slopes of heat, ion,
-// Ly-alpha), [6][kNXHII][nsteps] doubles, the only constants indexed per lane;
-// everything else is read uniformly from the constants in global memory.
+// Dynamic LDS: the folded shell coefficients cw, [kNXHII][nsteps][6] doubles,
+// the only constants indexed per lane (by the cell's x_e bracket); everything
+// else is read uniformly from the constants in global memory.
 constexpr size_t ts_cell_lds_bytes(int nsteps) { return sizeof(double) * 6 * kNXHII * (size_t)nsteps; }
 
-__device__ __forceinline__ void load_ts_tables(const TsDev& t, double* tabs)
+__device__ __forceinline__ void load_ts_coef(const TsDev& t, double* tabs)
 {
-  const int ns = t.nsteps, per = kNXHII * ns;
-  for (int k = threadIdx.x; k < 6 * per; k += blockDim.x) {
-    const int tb = k / per, rem = k % per, x = rem / ns, c = rem % ns;
-    const double(*src)[kTsMaxSteps] =
-      tb == 0 ? t.fheat : tb == 1 ? t.fion : tb == 2 ? t.flya : tb == 3 ? t.sheat : tb == 4 ? t.sion : t.slya;
-    tabs[k] = src[x][c];
+  const int ns = t.nsteps;
+  for (int k = threadIdx.x; k < 6 * kNXHII * ns; k += blockDim.x) {
+    const int lo = k / (6 * ns), rem = k % (6 * ns);
+    tabs[k] = t.cw[lo][rem / 6][rem % 6];
   }
 }
 
+// The cell's seven shell sums at its x_e bracket `lo` (ComputeTs.c:759-790,
+// evolveInt's 40-term sums): value and slope parts of heat, ion and Ly-alpha,
+// and the stellar Ly-alpha sum.  One fused multiply-add per term.
+struct ShellSums {
+  double hf, hs, xf, xs, lf, ls, st;
+};
+
+__device__ __forceinline__ ShellSums ts_shell_sums(const TsDev& t, const double* tabs, int lo, const float* shells,
+                                                   long ncell, long i)
+{
+  ShellSums r = {0, 0, 0, 0, 0, 0, 0};
+  const int ns = t.nsteps;
+  const double2* row = reinterpret_cast<const double2*>(tabs + (size_t)lo * ns * 6);
+#pragma unroll 4
+  for (int ct = 0; ct < ns; ++ct) {
+    const double v = (double)shells[(long)ct * ncell + i];
+    const double2 a = row[3 * ct], b = row[3 * ct + 1], c = row[3 * ct + 2];
+    r.hf = fma(v, a.x, r.hf);
+    r.xf = fma(v, a.y, r.xf);
+    r.lf = fma(v, b.x, r.lf);
+    r.hs = fma(v, b.y, r.hs);
+    r.xs = fma(v, c.x, r.xs);
+    r.ls = fma(v, c.y, r.ls);
+    r.st = fma(v, t.cst[ct], r.st);
+  }
+  return r;
+}
+
+// the table bracket of the cell's x_e (ComputeTs.c:759-790): lo and fr
+__device__ __forceinline__ int ts_bracket(const TsDev& t, float xe0, double* fr)
+{
+  double xHII_call = xe0;
+  if (xHII_call > t.XH[kNXHII - 1] * 0.999)
+    xHII_call = t.XH[kNXHII - 1] * 0.999;
+  else if (xHII_call < t.XH[0])
+    xHII_call = 1.001 * t.XH[0];
+  int lo = kNXHII - 1;
+  const float xc_f = (float)xHII_call;
+  while (xc_f < t.XH[lo]) lo--;
+  *fr = xHII_call - t.XH[lo];
+  return lo;
+}
+
+// One cell per thread (a grid-stride loop keeps the snapshot constants live
+// across iterations and halves the occupancy).
 __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, const float* __restrict__ shells,
                                                 const float* __restrict__ deltax, float* __restrict__ xe_prev,
                                                 float* __restrict__ Tk, float* __restrict__ TS, int N,
@@ -223,15 +274,8 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
   extern __shared__ double tabs[];
   __shared__ double red[kNT];
   const TsDev& t = *tg;
-  const int ns = t.nsteps;
-  load_ts_tables(t, tabs);
+  load_ts_coef(t, tabs);
   __syncthreads();
-  const double* fheat = tabs;
-  const double* fion = tabs + kNXHII * ns;
-  const double* flya = tabs + 2 * kNXHII * ns;
-  const double* sheat = tabs + 3 * kNXHII * ns;
-  const double* sion = tabs + 4 * kNXHII * ns;
-  const double* slya = tabs + 5 * kNXHII * ns;
   const long ncell = (long)N * N * N;
   double sJ = 0, sxa = 0, sXh = 0, sXi = 0, sTS = 0, sTK = 0, sxe = 0;
   const long i = (long)blockIdx.x * kNT + threadIdx.x;
@@ -242,44 +286,16 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
     const float dl = deltax[ip];
     const float xe0 = xe_prev[ip];
     const double x_e = xe0, T = Tk[i];
-    // table interpolation at the cell's x_e (:759-790); the bracket is the same for every shell
-    double xHII_call = xe0;
-    if (xHII_call > t.XH[kNXHII - 1] * 0.999)
-      xHII_call = t.XH[kNXHII - 1] * 0.999;
-    else if (xHII_call < t.XH[0])
-      xHII_call = 1.001 * t.XH[0];
-    int lo = kNXHII - 1;
-    const float xc_f = (float)xHII_call;
-    while (xc_f < t.XH[lo]) lo--;
-    const double fr = xHII_call - t.XH[lo];
+    double fr;
+    const int lo = ts_bracket(t, xe0, &fr);
     const double n_b = t.NBP3 * (1 + dl);
     double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
     if (!t.no_light) {
-#pragma unroll 4
-      for (int ct = 0; ct < t.nsteps; ++ct) {
-        const float sv = shells[(long)ct * ncell + i];
-        const double SFR = (double)sv * t.sfr_factor;
-        const int at = lo * ns + ct;
-        double fh = sheat[at];
-        fh *= fr;
-        fh += fheat[at];
-        double fi = sion[at];
-        fi *= fr;
-        fi += fion[at];
-        double fl = slya[at];
-        fl *= fr;
-        fl += flya[at];
-        const double dt_dzpp = t.dt_dzpp[ct], dzpp = t.dzpp[ct];
-        const double zpp_integrand = SFR * t.zpp_pow[ct];
-        dxheat += dt_dzpp * dzpp * zpp_integrand * fh;
-        dxion += dt_dzpp * dzpp * zpp_integrand * fi;
-        dxlya += dt_dzpp * dzpp * zpp_integrand * fl;
-        dstarlya += SFR * t.P2 * t.zpp1[ct] * t.sum_lyn[ct] * dt_dzpp * dzpp;
-      }
-      dxheat *= t.const_zp_prefactor;
-      dxion *= t.const_zp_prefactor;
-      dxlya *= t.const_zp_prefactor * n_b;
-      dstarlya *= (2.9979e10 / (4. * M_PI)) / (1.6726e-24 / 1.989e33);
+      const ShellSums S = ts_shell_sums(t, tabs, lo, shells, ncell, i);
+      dxheat = (S.hf + fr * S.hs) * t.const_zp_prefactor;
+      dxion = (S.xf + fr * S.xs) * t.const_zp_prefactor;
+      dxlya = (S.lf + fr * S.ls) * (t.const_zp_prefactor * n_b);
+      dstarlya = S.st * ((2.9979e10 / (4. * M_PI)) / (1.6726e-24 / 1.989e33));
     }
     const double sink = dev_alpha_A(T) * 2.0 * x_e * x_e * t.f_H * n_b;
     const double dxe_dzp = t.dt_dzp * (dxion - sink);
@@ -336,19 +352,11 @@ __global__ __launch_bounds__(kNT) void k_ts_homogeneous(const TsDev* __restrict_
                                                        float* __restrict__ TS, int N, float xe0, float T0,
                                                        double* __restrict__ partials)
 {
-  __shared__ TsDev t;
   __shared__ double red[kNT];
-  {
-    const int words = (int)(sizeof(TsDev) / 4);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(tg);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&t);
-    for (int i = threadIdx.x; i < words; i += kNT) dst[i] = src[i];
-  }
-  __syncthreads();
+  const TsDev& t = *tg;
   const long ncell = (long)N * N * N;
-  const long i = (long)blockIdx.x * kNT + threadIdx.x;
   double sTS = 0, sTK = 0, sxe = 0;
-  if (i < ncell) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < ncell; i += (long)gridDim.x * kNT) {
     const long r = i / N;
     const int z = (int)(i % N);
     const long ip = r * (N + 2) + z;
@@ -357,9 +365,9 @@ __global__ __launch_bounds__(kNT) void k_ts_homogeneous(const TsDev* __restrict_
     xe_prev[ip] = xe0;
     Tk[i] = T0;
     TS[i] = ts;
-    sTS = ts;
-    sTK = T0;
-    sxe = xe0;
+    sTS += ts;
+    sTK += T0;
+    sxe += xe0;
   }
   double* p = partials + (long)blockIdx.x * 7;
   const double v4 = block_sum<kNT>(sTS, red), v5 = block_sum<kNT>(sTK, red), v6 = block_sum<kNT>(sxe, red);
