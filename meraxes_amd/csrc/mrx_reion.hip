@@ -798,6 +798,9 @@ void fill_ts_dev(TsDev& t, const mrx_ts_data* d, const mrx_ts_snap* s, const mrx
     }
     t.cst[c] = t.sfr_factor * P2 * t.zpp1[c] * t.sum_lyn[c] * t.dt_dzpp[c] * t.dzpp[c];
   }
+  for (int x = 0; x < kNXHII; ++x)
+    for (int c = 0; c < s->nsteps; ++c)
+      for (int k = 0; k < 6; ++k) t.cwp[((size_t)x * s->nsteps + c) * 6 + k] = t.cw[x][c][k];
   t.zp_f = zp;
   t.dzp = s->dzp;
   t.dt_dzp = s->dt_dzp;

@@ -623,6 +623,35 @@ __device__ __forceinline__ double block_sum(double v, double* red)
   return r;
 }
 
+// Block sums of NV values at once: a butterfly over each wave's 64 lanes
+// (lane i adds lane i^m; addition commutes, so every lane holds the same
+// bits), then the NT/64 wave sums in wave order.  Two barriers in all.  On
+// return every thread holds the block's sums in v; red holds NV * NT / 64
+// doubles.
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum_n(double (&v)[NV], double* red)
+{
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v[k] += __shfl_xor(v[k], m, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[wave * NV + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double t = red[k];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) t += red[w * NV + k];
+    v[k] = t;
+  }
+  __syncthreads();
+}
+
 // partials[nb][nv] -> out[nv], fixed order (deterministic); block b reduces
 // the b-th such set (one box of a batch)
 __global__ __launch_bounds__(kNT) void k_reduce_partials(const double* __restrict__ partials, int nb, int nv,

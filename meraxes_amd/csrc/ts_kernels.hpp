@@ -30,6 +30,11 @@ struct KappaSpline {
   double x[27], y[27], b[27], c[27], d[27];
 };
 
+// the LDS copy of the packed shell coefficients: kTsCoefRounds 16-byte loads
+// per lane over kTsCoefLanes lanes cover 3 kNXHII kTsMaxSteps double2
+constexpr int kTsCoefLanes = 256;
+constexpr int kTsCoefRounds = (3 * kNXHII * kTsMaxSteps + kTsCoefLanes - 1) / kTsCoefLanes;
+
 // Everything the per-cell kernel needs, uploaded once per snapshot.
 struct TsDev {
   int nsteps;
@@ -71,6 +76,10 @@ struct TsDev {
   // stellar Ly-alpha sum
   double cw[kNXHII][kTsMaxSteps][6];
   double cst[kTsMaxSteps];
+  // cw packed for the actual shell count: [lo][ct][6] with stride nsteps, the
+  // per-cell kernels' LDS image (6 kNXHII nsteps doubles), 16-byte aligned and
+  // padded to whole rounds of the copy (kTsCoefRounds loads of 16 bytes per lane)
+  alignas(16) double cwp[2 * kTsCoefRounds * kTsCoefLanes];
 };
 
 #pragma clang fp contract(off)
@@ -159,15 +168,18 @@ __device__ __forceinline__ R get_Ts_t(const TsDev& t, R dl, R TK, R xe, R Jalpha
     const R Sden = 1.0 + 2.98394 * xi + 1.53583 * xi * xi + 3.85289 * xi * xi * xi;
     const R invTK = 1.0 / TK, ceTK = 0.405535 / TK, xcTK = xc / TK;
     const double invTrad = 1.0 / Trad;
+    // the loop body's divisions by TS, TK and Sden as products with their
+    // reciprocals, and xa / Tceff as xa * ce (Tceff = 1 / ce): three divisions
+    // per iteration instead of ten, the same values to rounding
+    const R aTK = 0.401403 * invTK, bTK = 0.336463 * invTK * invTK, invSden = 1.0 / Sden;
+    const R XJ = t.XA * Jalpha;
     while (fabs(pval(TS) - pval(TSold)) / pval(TS) > 1.0e-3) {
       TSold = TS;
-      R S = S0 - 0.401403 / TS / TK;
-      S += 0.336463 / TS / TK / TK;
-      S /= Sden;
-      xa = t.XA * S * Jalpha;
-      const R ce = invTK + ceTK * (1.0 / TS - invTK);
-      const R Tceff = 1.0 / ce;
-      const R TSinv = (invTrad + xa / Tceff + xcTK) / (1.0 + xa + xc);
+      const R invTS = 1.0 / TS;
+      const R S = (S0 - aTK * invTS + bTK * invTS) * invSden;
+      xa = XJ * S;
+      const R ce = invTK + ceTK * (invTS - invTK);
+      const R TSinv = (invTrad + xa * ce + xcTK) / (1.0 + xa + xc);
       TS = 1.0 / TSinv;
     }
     *xa_out = xa;
@@ -212,12 +224,21 @@ __device__ __forceinline__ double dev_alpha_A(double T) { return alpha_A_t<doubl
 // else is read uniformly from the constants in global memory.
 constexpr size_t ts_cell_lds_bytes(int nsteps) { return sizeof(double) * 6 * kNXHII * (size_t)nsteps; }
 
+// Copies the packed coefficients into LDS (3 kNXHII nsteps double2), 16 bytes
+// per lane and load, a lane's loads issued together; blockDim.x == kTsCoefLanes.
 __device__ __forceinline__ void load_ts_coef(const TsDev& t, double* tabs)
 {
-  const int ns = t.nsteps;
-  for (int k = threadIdx.x; k < 6 * kNXHII * ns; k += blockDim.x) {
-    const int lo = k / (6 * ns), rem = k % (6 * ns);
-    tabs[k] = t.cw[lo][rem / 6][rem % 6];
+  static_assert(kTsCoefLanes == kNT, "one copy lane per thread");
+  const int n2 = 3 * kNXHII * t.nsteps;
+  const double2* src = reinterpret_cast<const double2*>(t.cwp);
+  double2* dst = reinterpret_cast<double2*>(tabs);
+  double2 v[kTsCoefRounds];
+#pragma unroll
+  for (int u = 0; u < kTsCoefRounds; ++u) v[u] = src[u * kTsCoefLanes + threadIdx.x];  // in the padding
+#pragma unroll
+  for (int u = 0; u < kTsCoefRounds; ++u) {
+    const int k = u * kTsCoefLanes + (int)threadIdx.x;
+    if (k < n2) dst[k] = v[u];
   }
 }
 
@@ -234,17 +255,34 @@ __device__ __forceinline__ ShellSums ts_shell_sums(const TsDev& t, const double*
   ShellSums r = {0, 0, 0, 0, 0, 0, 0};
   const int ns = t.nsteps;
   const double2* row = reinterpret_cast<const double2*>(tabs + (size_t)lo * ns * 6);
-#pragma unroll 4
-  for (int ct = 0; ct < ns; ++ct) {
-    const double v = (double)shells[(long)ct * ncell + i];
-    const double2 a = row[3 * ct], b = row[3 * ct + 1], c = row[3 * ct + 2];
-    r.hf = fma(v, a.x, r.hf);
-    r.xf = fma(v, a.y, r.xf);
-    r.lf = fma(v, b.x, r.lf);
-    r.hs = fma(v, b.y, r.hs);
-    r.xs = fma(v, c.x, r.xs);
-    r.ls = fma(v, c.y, r.ls);
-    r.st = fma(v, t.cst[ct], r.st);
+  // shells in groups of 4, the next group's loads in flight while this one is summed
+  constexpr int kG = 4;
+  float nxt[kG];
+#pragma unroll
+  for (int u = 0; u < kG; ++u) nxt[u] = u < ns ? shells[(long)u * ncell + i] : 0.0f;
+  for (int c0 = 0; c0 < ns; c0 += kG) {
+    float cur[kG];
+#pragma unroll
+    for (int u = 0; u < kG; ++u) cur[u] = nxt[u];
+#pragma unroll
+    for (int u = 0; u < kG; ++u) {
+      const int cn = c0 + kG + u;
+      nxt[u] = cn < ns ? shells[(long)cn * ncell + i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kG; ++u) {
+      const int ct = c0 + u;
+      if (ct >= ns) break;
+      const double v = (double)cur[u];
+      const double2 a = row[3 * ct], b = row[3 * ct + 1], c = row[3 * ct + 2];
+      r.hf = fma(v, a.x, r.hf);
+      r.xf = fma(v, a.y, r.xf);
+      r.lf = fma(v, b.x, r.lf);
+      r.hs = fma(v, b.y, r.hs);
+      r.xs = fma(v, c.x, r.xs);
+      r.ls = fma(v, c.y, r.ls);
+      r.st = fma(v, t.cst[ct], r.st);
+    }
   }
   return r;
 }
@@ -331,17 +369,11 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
     sxe = xn;
   }
   double* p = partials + (long)blockIdx.x * 7;
-  const double v0 = block_sum<kNT>(sJ, red), v1 = block_sum<kNT>(sxa, red), v2 = block_sum<kNT>(sXh, red);
-  const double v3 = block_sum<kNT>(sXi, red), v4 = block_sum<kNT>(sTS, red), v5 = block_sum<kNT>(sTK, red);
-  const double v6 = block_sum<kNT>(sxe, red);
+  double v[7] = {sJ, sxa, sXh, sXi, sTS, sTK, sxe};
+  block_sum_n<kNT, 7>(v, red);
   if (threadIdx.x == 0) {
-    p[0] = v0;
-    p[1] = v1;
-    p[2] = v2;
-    p[3] = v3;
-    p[4] = v4;
-    p[5] = v5;
-    p[6] = v6;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) p[k] = v[k];
   }
 }
 
