@@ -51,11 +51,13 @@ def model_bytes_hii(N, F, S, uvb):
     return F * 6 * Bc + S * (F * 6 * Bc + (4 * F + W) * Nr)
 
 
-def kernel_bytes(name, N, F):
+def kernel_bytes(name, N, F, radii=1.0):
     """Algorithmic HBM bytes of one launch of a kernel (packed spectra: 8*N*N*N/2 bytes).
 
     Names with a ts_ prefix are launches of the spin-temperature shell loop,
     one field each; the others are the bubble finder's, F fields each.
+    ``radii`` is the average number of ladder radii one ix_filter_k launch
+    covers (it reads each field's spectrum once and writes one per radius).
     """
     if name.startswith("ts_"):
         name, F = name[3:], 1
@@ -75,6 +77,7 @@ def kernel_bytes(name, N, F):
         "colpass_x_fwd": 2 * spec,
         # window + inverse x pass: read unfiltered (+ Nyquist plane) and write per field
         "ix_filter": F * (2 * spec + 8 * N * N),
+        "ix_filter_k": F * ((1 + radii) * spec + 8 * N * N),
         # inverse z pass + threshold: read each field's spectrum, 1 byte of step state per cell
         "c2r_threshold": F * spec + 2 * Nr,
         "r2c_z": 4 * N * N * (N + 2) + spec,
@@ -668,13 +671,16 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][0])
     name, (ms_tot, cnt) = dom
     avg_s = ms_tot / cnt / 1e3
-    b = kernel_bytes(name, N, F)
+    # radii per multi-radius x pass: the S - 1 filtered radii over its launches
+    nk = prof.get("ix_filter_k", (0, 0))[1] // args.steps
+    radii = (S - 1) / nk if nk else 1.0
+    b = kernel_bytes(name, N, F, radii)
     roof = {"kernel": name, "bound": "hbm", "achieved": (b / avg_s / 1e9) if b else None, "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": (b / avg_s / HBM_PEAK) if b else None, "traffic": measured_traffic(name, N),
             "avg_launch_us": avg_s * 1e6, "bytes_per_launch": b}
     per_kernel = {}
     for k, (ms, n) in prof.items():
-        kb = kernel_bytes(k, N, F)
+        kb = kernel_bytes(k, N, F, radii)
         per_kernel[k] = {"ms_per_step": ms / args.steps, "launches_per_step": n // args.steps,
                          "us_per_launch": ms / n * 1e3,
                          "GB_s": (kb * n / (ms / 1e3) / 1e9) if kb else None,

@@ -38,6 +38,14 @@ constexpr double kLFactor = 0.620350491;
 constexpr int kMaxSteps = 255;  // the per-cell step index is one byte
 }  // namespace
 
+// Radii per inverse x pass of the bubble ladder (1..kLadderRadiiMax); the
+// MRX_LADDER_RADII environment variable, read when a context is created,
+// overrides it.  Two radii per pass bring the x pass to the y pass's rate at
+// 512^3 (more gain nothing); the second radius's spectra cost nfield
+// N^2 (N/2) float2 of device memory.
+constexpr int kLadderRadii = 2;
+constexpr int kLadderRadiiMax = 8;
+
 struct mrx_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
@@ -78,8 +86,13 @@ struct mrx_ctx {
   size_t ts_work_cap = 0;
   float* ts_shells = nullptr;
   size_t ts_shells_cap = 0;
-  float* wtab = nullptr;  // window table of the current radius, (N/2+1)^3 floats
+  float* wtab = nullptr;  // window tables of the current radii, (N/2+1)^3 floats each
   size_t wtab_cap = 0;
+  // the bubble ladder's inverse x pass covers ladder_radii radii per launch
+  // (tabulated windows): their x-pass outputs, ladder_radii x nfield spectra
+  int ladder_radii = kLadderRadii;
+  float2* TK = nullptr;
+  size_t TK_cap = 0;  // float2
   float ts_h_xe0 = 0.f, ts_h_T0 = 0.f;  // RECFAST x_e, T_K of the homogeneous branch
   // adjoint work buffers
   float* adj_work = nullptr;
@@ -246,6 +259,24 @@ int ensure_partials(mrx_ctx* ctx, size_t n)
   ctx->partials_cap = 0;
   HIP_TRY(hipMalloc(&ctx->partials, sizeof(double) * n));
   ctx->partials_cap = n;
+  return MRX_OK;
+}
+
+// The ladder's multi-radius spectra (float2 elements); a failed allocation is
+// not an error (the ladder then runs one radius per x pass).
+int ensure_tk(mrx_ctx* ctx, size_t n)
+{
+  if (ctx->TK_cap >= n) return MRX_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ctx->TK) (void)hipFree(ctx->TK);
+  ctx->TK = nullptr;
+  ctx->TK_cap = 0;
+  if (hipMalloc(&ctx->TK, sizeof(float2) * n) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->TK = nullptr;
+    return MRX_ENOMEM;
+  }
+  ctx->TK_cap = n;
   return MRX_OK;
 }
 
@@ -446,6 +477,36 @@ struct Pipe {
     return check_launch(ctx, "k_colpass y inv");
   }
 
+  // The inverse x and y passes of K ladder radii with tabulated windows ws[k]
+  // (not sharp-k): one x pass reads U once and writes the K spectra TK + k
+  // tstride, then each radius's y pass runs in place.  Per radius the spectra
+  // are inverse_xy's.
+  static int inverse_xy_k(mrx_ctx* ctx, const float2* U, const float2* V, float2* TK, long tstride, int nfield,
+                          const WinArgs* ws, int K)
+  {
+    const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
+    MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt * K));
+    for (int k = 0; k < K; ++k) {
+      PROF("window_table");
+      k_window_table<N, Cfg<N>::TW><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->wtab + k * nt,
+                                                                                              ws[k]);
+    }
+    MRX_TRY(check_launch(ctx, "k_window_table"));
+    {
+      PROF("ix_filter_k");
+      k_ix_filter_k<N><<<N * (NH / Cfg<N>::TW), Cfg<N>::COL_NT, 0, ctx->stream>>>(
+        U, V, TK, ctx->tw, ctx->wtab, nt, nc, (long)N * N, tstride, nfield, K);
+    }
+    MRX_TRY(check_launch(ctx, "k_ix_filter_k"));
+    const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
+    for (int k = 0; k < K; ++k) {
+      PROF("colpass_y_inv");
+      k_colpass<N, true><<<gy, Cfg<N>::COL_NT, 0, ctx->stream>>>(TK + k * tstride, ctx->tw, nc, (long)N * NH, NH,
+                                                                  0.0);
+    }
+    return check_launch(ctx, "k_colpass y inv");
+  }
+
   static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp, int pitch = N + 2,
                    int kzlive = NH)
   {
@@ -456,14 +517,14 @@ struct Pipe {
   }
 
   // the ladder's last pass; fields deltax, stars, sfr (+ x_e) (+ N_rec when rec)
-  static int threshold(mrx_ctx* ctx, const float* deltax, float* xH, float* rb, float* J21, float* zin, float* J21i,
-                       float* Gamma12, bool rec, const StepArgs& a)
+  static int threshold(mrx_ctx* ctx, const float2* Tsrc, const float* deltax, float* xH, float* rb, float* J21,
+                       float* zin, float* J21i, float* Gamma12, bool rec, const StepArgs& a)
   {
     const int nb = N * N / ThrCfg<N>::RZ;
     constexpr int NT = ThrCfg<N>::NT;
     PROF("c2r_threshold");
 #define MRX_THR(NF_, REC_)                                                                                          \
-  k_c2r_threshold<N, NF_, REC_><<<nb, NT, 0, ctx->stream>>>(ctx->T, nc, ctx->tw, deltax, ctx->state, xH, rb, J21, \
+  k_c2r_threshold<N, NF_, REC_><<<nb, NT, 0, ctx->stream>>>(Tsrc, nc, ctx->tw, deltax, ctx->state, xH, rb, J21,   \
                                                            zin, J21i, Gamma12, ctx->Rtab, ctx->partials, a)
     if (!rec && a.nfield == 4)
       MRX_THR(4, false);
@@ -634,28 +695,50 @@ struct HiiRun {
       a.use_xe = p->include_spin_temp;
       a.nfield = nfield;
       const double z = p->redshift;
-      for (int s = 0; s < S; ++s) {
+      // the filtered radii go through the x pass ladder_radii at a time
+      // (tabulated windows; KL = 0: sharp-k, one radius per pass over its live
+      // tiles) when the spectra of that many radii fit, else one at a time
+      int KL = (p->filter_type == MRX_FILTER_SHARPK) ? 0 : ctx->ladder_radii;
+      const long tstride = (long)nfield * P::nc;
+      if (KL > 1 && ensure_tk(ctx, (size_t)KL * tstride) != MRX_OK) KL = 1;
+      for (int s = 0; s < S;) {
         const bool last = (s == S - 1);
-        const WinArgs w = win_args(box_size, (float)R[s], p->filter_type, last ? 0 : 1);
-        MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, nfield, w, a.kzlive));
-        a.J21c = (1.0 + z) * (1.0 + z) / (4.0 * M_PI) * p->alpha_uv * kPlanck * 1e21 * R[s] * p->unit_length_cm *
-                 p->nion_phot_per_bary / kProtonMass * p->unit_mass_g / pow(p->unit_length_cm, 3) / p->unit_time_s;
-        a.M_mean = RtoM(p, R[s]);
-        a.R_cubed = R[s] * R[s] * R[s];
-        a.kfc = (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume / a.M_mean;
-        if (rec) {
-          // Gamma_R_prefactor (find_HII_bubbles.c:325-334): R h^-1 to R, pixel volume (Mpc/h)^3 to cm^3
-          double gp = (1.0 + z) * (1.0 + z) * R[s] * (p->unit_length_cm / p->hubble_h) * kSigmaHI * p->alpha_uv /
-                      (p->alpha_uv + 2.75) / 1.0e-12;
-          gp *= (p->unit_mass_g / p->unit_time_s) * pow(p->unit_length_cm / p->hubble_h, -3.) *
-                p->nion_phot_per_bary / kProtonMass;
-          a.gamma_pref = gp;
-        } else {
-          a.gamma_pref = 0.0;
+        int K = 1;
+        const float2* Tsrc = ctx->T;
+        if (!last && KL >= 1) {
+          K = std::min(KL, S - 1 - s);
+          WinArgs ws[kLadderRadiiMax];
+          for (int k = 0; k < K; ++k) ws[k] = win_args(box_size, (float)R[s + k], p->filter_type, 1);
+          float2* TK = KL > 1 ? ctx->TK : ctx->T;
+          MRX_TRY(P::inverse_xy_k(ctx, ctx->U, ctx->V, TK, tstride, nfield, ws, K));
+          a.kzlive = N / 2;
         }
-        a.step = s;
-        a.last = last ? 1 : 0;
-        MRX_TRY(P::threshold(ctx, deltax, out[0], out[1], out[2], out[3], out[4], out[5], rec, a));
+        for (int k = 0; k < K; ++k, ++s) {
+          if (KL >= 1 && !last) {
+            Tsrc = (KL > 1 ? ctx->TK : ctx->T) + k * tstride;
+          } else {
+            const WinArgs w = win_args(box_size, (float)R[s], p->filter_type, last ? 0 : 1);
+            MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, nfield, w, a.kzlive));
+          }
+          a.J21c = (1.0 + z) * (1.0 + z) / (4.0 * M_PI) * p->alpha_uv * kPlanck * 1e21 * R[s] * p->unit_length_cm *
+                   p->nion_phot_per_bary / kProtonMass * p->unit_mass_g / pow(p->unit_length_cm, 3) / p->unit_time_s;
+          a.M_mean = RtoM(p, R[s]);
+          a.R_cubed = R[s] * R[s] * R[s];
+          a.kfc = (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume / a.M_mean;
+          if (rec) {
+            // Gamma_R_prefactor (find_HII_bubbles.c:325-334): R h^-1 to R, pixel volume (Mpc/h)^3 to cm^3
+            double gp = (1.0 + z) * (1.0 + z) * R[s] * (p->unit_length_cm / p->hubble_h) * kSigmaHI * p->alpha_uv /
+                        (p->alpha_uv + 2.75) / 1.0e-12;
+            gp *= (p->unit_mass_g / p->unit_time_s) * pow(p->unit_length_cm / p->hubble_h, -3.) *
+                  p->nion_phot_per_bary / kProtonMass;
+            a.gamma_pref = gp;
+          } else {
+            a.gamma_pref = 0.0;
+          }
+          a.step = s;
+          a.last = last ? 1 : 0;
+          MRX_TRY(P::threshold(ctx, Tsrc, deltax, out[0], out[1], out[2], out[3], out[4], out[5], rec, a));
+        }
       }
       if (rec) {
         // N_rec += dNrec after the ladder (:472-477)
@@ -1445,6 +1528,10 @@ int mrx_init(int device, mrx_ctx** out)
     return MRX_EDEVICE;
   }
   ctx->stream = ctx->own_stream;
+  if (const char* e = getenv("MRX_LADDER_RADII")) {
+    const int k = atoi(e);
+    if (k >= 1 && k <= kLadderRadiiMax) ctx->ladder_radii = k;
+  }
   *out = ctx;
   return MRX_OK;
 }
@@ -1467,6 +1554,7 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->ts_work);
   dfree(ctx->ts_shells);
   dfree(ctx->wtab);
+  if (ctx->TK) (void)hipFree(ctx->TK);
   dfree(ctx->adj_work);
   dfree(ctx->adj_gS);
   dfree(ctx->tbv);

@@ -499,6 +499,73 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
   }
 }
 
+// Inverse x pass of K consecutive ladder radii at once, for the tabulated
+// windows (top-hat, Gaussian): for each field, each radius k < K windows the
+// field's tile of the unfiltered spectrum U with table k (Wt + k wstride,
+// k_window_table's layout), transforms it and writes it to T + k tstride.  Per
+// radius the result is k_ix_filter<N, kWinTable>'s, operation for operation.
+// A workgroup reads its tile of U once per radius, K times within a few
+// microseconds, so the repeats are served by the caches and HBM sees about
+// one read of U per K radii.
+template <int N, int TW = Cfg<N>::TW>
+__global__ __launch_bounds__((TileMap<N, TW>::NT)) __attribute__((amdgpu_waves_per_eu(4))) void k_ix_filter_k(
+  const float2* __restrict__ U, const float2* __restrict__ V, float2* __restrict__ T,
+  const float2* __restrict__ twN, const float* __restrict__ Wt, long wstride, long fstride, long vstride,
+  long tstride, int nfield, int K)
+{
+  using M = TileMap<N, TW>;  // kColNT threads, as k_colpass
+  constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
+  constexpr int NH = N / 2, NH1 = NH + 1;
+  constexpr unsigned rs = (unsigned)N * NH;
+  __shared__ float2 s[N * SP];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  const int c = tid % TW, n0 = tid / TW;
+  const int ky = blockIdx.x / NKZB, b = blockIdx.x % NKZB;
+  const int aky = ky <= NH ? ky : N - ky;
+  const long base = (long)ky * NH + (long)b * TW + (unsigned)n0 * rs + c;
+  const float* wr0 = Wt + ((long)aky * NKZB + b) * NH1 * TW + c;
+  for (int i = tid; i < N; i += NT) tw[i] = twN[i];
+  const bool nyq = (b == 0);  // this tile holds the kz=0 slot, which carries the Nyquist plane
+  const int items = nfield * K;
+  for (int j = 0; j < items; ++j) {
+    const int f = j / K, k = j - f * K;
+    {
+      // the tile and this radius's window; nothing is held in registers across
+      // the transform, the two workgroups of a CU hide each other's loads
+      const float2* u = U + f * fstride + base;
+      float2 r[EPT];
+#pragma unroll
+      for (int i = 0; i < EPT; ++i) r[i] = u[(unsigned)(i * RPI) * rs];
+      const float* wr = wr0 + k * wstride;
+#pragma unroll
+      for (int i = 0; i < EPT; ++i) {
+        const int n = n0 + i * RPI;
+        const int an = n <= NH ? n : N - n;
+        s[n * SP + c] = cscale(r[i], wr[(unsigned)an * TW]);
+      }
+    }
+    if (nyq) {
+      // the kz=0 slot carries A*W0 + i*(B*WN), B the Nyquist plane
+      __syncthreads();
+      const float2* vp = V + f * vstride + (long)ky * N;
+      const float* wn = Wt + k * wstride + (long)NH1 * NH * NH1 + (long)aky * NH1;
+      for (int n = tid; n < N; n += NT) {
+        const int an = n <= NH ? n : N - n;
+        const float2 nb = cscale(vp[n], wn[an]);
+        const float2 v = s[n * SP];
+        s[n * SP] = make_float2(v.x - nb.y, v.y + nb.x);
+      }
+    }
+    __syncthreads();
+    fft_lds<N, TW, NT, true, SP, 1, true, 1, false, 0>(s, tw, tid);
+    float2* t = T + k * tstride + f * fstride + base;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) t[(unsigned)(i * RPI) * rs] = s[(n0 + i * RPI) * SP + c];
+    __syncthreads();
+  }
+}
+
 // Inverse pass over the live tiles of a sharp-k filtered spectrum left by
 // k_ix_filter<kWinSharpK>: kz blocks below nkzb only (grid N x nkzb per
 // field, tile t = line t / nkzb, kz block t % nkzb), and rows n with
