@@ -884,6 +884,7 @@ void fill_ts_dev(TsDev& t, const mrx_ts_data* d, const mrx_ts_snap* s, const mrx
   for (int x = 0; x < kNXHII; ++x)
     for (int c = 0; c < s->nsteps; ++c)
       for (int k = 0; k < 6; ++k) t.cwp[((size_t)x * s->nsteps + c) * 6 + k] = t.cw[x][c][k];
+  for (int c = 0; c < s->nsteps; ++c) t.cwp[(size_t)6 * kNXHII * s->nsteps + c] = t.cst[c];
   t.zp_f = zp;
   t.dzp = s->dzp;
   t.dt_dzp = s->dt_dzp;

@@ -30,10 +30,11 @@ struct KappaSpline {
   double x[27], y[27], b[27], c[27], d[27];
 };
 
-// the LDS copy of the packed shell coefficients: kTsCoefRounds 16-byte loads
-// per lane over kTsCoefLanes lanes cover 3 kNXHII kTsMaxSteps double2
+// the LDS copy of the packed shell coefficients (the [lo][ct][6] table, then
+// cst): kTsCoefRounds 16-byte loads per lane over kTsCoefLanes lanes cover
+// 3 kNXHII kTsMaxSteps + kTsMaxSteps / 2 double2
 constexpr int kTsCoefLanes = 256;
-constexpr int kTsCoefRounds = (3 * kNXHII * kTsMaxSteps + kTsCoefLanes - 1) / kTsCoefLanes;
+constexpr int kTsCoefRounds = (3 * kNXHII * kTsMaxSteps + kTsMaxSteps / 2 + kTsCoefLanes - 1) / kTsCoefLanes;
 
 // Everything the per-cell kernel needs, uploaded once per snapshot.
 struct TsDev {
@@ -76,9 +77,10 @@ struct TsDev {
   // stellar Ly-alpha sum
   double cw[kNXHII][kTsMaxSteps][6];
   double cst[kTsMaxSteps];
-  // cw packed for the actual shell count: [lo][ct][6] with stride nsteps, the
-  // per-cell kernels' LDS image (6 kNXHII nsteps doubles), 16-byte aligned and
-  // padded to whole rounds of the copy (kTsCoefRounds loads of 16 bytes per lane)
+  // the per-cell kernels' LDS image, 16-byte aligned and padded to whole
+  // rounds of the copy (kTsCoefRounds loads of 16 bytes per lane): cw packed
+  // for the actual shell count, [lo][ct][6] with stride nsteps (6 kNXHII
+  // nsteps doubles), then cst[0 .. nsteps)
   alignas(16) double cwp[2 * kTsCoefRounds * kTsCoefLanes];
 };
 
@@ -220,16 +222,21 @@ __device__ __forceinline__ double dev_alpha_A(double T) { return alpha_A_t<doubl
 // Per-cell spin-temperature update (ComputeTs.c:743-900); 7 block partial sums:
 // J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e.
 // Dynamic LDS: the folded shell coefficients cw, [kNXHII][nsteps][6] doubles,
-// the only constants indexed per lane (by the cell's x_e bracket); everything
-// else is read uniformly from the constants in global memory.
-constexpr size_t ts_cell_lds_bytes(int nsteps) { return sizeof(double) * 6 * kNXHII * (size_t)nsteps; }
+// the only constants indexed per lane (by the cell's x_e bracket), then the
+// stellar Ly-alpha coefficients cst[nsteps] (read in the shell loop, where a
+// scalar load would stall every group of shells); everything else is read
+// uniformly from the constants in global memory.
+constexpr size_t ts_cell_lds_bytes(int nsteps)
+{
+  return sizeof(double) * ((6 * kNXHII + 1) * (size_t)nsteps + (size_t)(nsteps & 1));
+}
 
 // Copies the packed coefficients into LDS (3 kNXHII nsteps double2), 16 bytes
 // per lane and load, a lane's loads issued together; blockDim.x == kTsCoefLanes.
 __device__ __forceinline__ void load_ts_coef(const TsDev& t, double* tabs)
 {
   static_assert(kTsCoefLanes == kNT, "one copy lane per thread");
-  const int n2 = 3 * kNXHII * t.nsteps;
+  const int n2 = 3 * kNXHII * t.nsteps + (t.nsteps + 1) / 2;
   const double2* src = reinterpret_cast<const double2*>(t.cwp);
   double2* dst = reinterpret_cast<double2*>(tabs);
   double2 v[kTsCoefRounds];
@@ -255,8 +262,10 @@ __device__ __forceinline__ ShellSums ts_shell_sums(const TsDev& t, const double*
   ShellSums r = {0, 0, 0, 0, 0, 0, 0};
   const int ns = t.nsteps;
   const double2* row = reinterpret_cast<const double2*>(tabs + (size_t)lo * ns * 6);
-  // shells in groups of 4, the next group's loads in flight while this one is summed
-  constexpr int kG = 4;
+  const double* cst = tabs + (size_t)6 * kNXHII * ns;
+  // shells in groups of kG, the next group's loads in flight while this one
+  // is summed (the loads' latency is the kernel's main wait)
+  constexpr int kG = 16;
   float nxt[kG];
 #pragma unroll
   for (int u = 0; u < kG; ++u) nxt[u] = u < ns ? shells[(long)u * ncell + i] : 0.0f;
@@ -281,7 +290,7 @@ __device__ __forceinline__ ShellSums ts_shell_sums(const TsDev& t, const double*
       r.hs = fma(v, b.y, r.hs);
       r.xs = fma(v, c.x, r.xs);
       r.ls = fma(v, c.y, r.ls);
-      r.st = fma(v, t.cst[ct], r.st);
+      r.st = fma(v, cst[ct], r.st);
     }
   }
   return r;
@@ -310,7 +319,7 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
                                                 double* __restrict__ partials)
 {
   extern __shared__ double tabs[];
-  __shared__ double red[kNT];
+  __shared__ double red[7 * kNT / 64];
   const TsDev& t = *tg;
   load_ts_coef(t, tabs);
   __syncthreads();
