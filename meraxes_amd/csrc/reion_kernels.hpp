@@ -837,6 +837,13 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   __shared__ double red[NT];
   const int tid = threadIdx.x;
   const long row0 = (long)blockIdx.x * RZ;
+  // the cells' ladder state, loaded with the spectra (the barriers of the
+  // transform would otherwise leave its latency exposed)
+  uint32_t st_pre[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j)
+    st_pre[j] = (a.step == 0) ? 0xFFFFFFFFu
+                              : *reinterpret_cast<const uint32_t*>(state + row0 * N + 4 * (tid + j * NT));
   {
     float2 v[EPT];
 #pragma unroll
@@ -875,7 +882,7 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   for (int j = 0; j < G; ++j) {
     const int e0 = 4 * (tid + j * NT);
     const long ir0 = cell0 + e0;
-    const uint32_t st_old = (a.step == 0) ? 0xFFFFFFFFu : *reinterpret_cast<const uint32_t*>(state + ir0);
+    const uint32_t st_old = st_pre[j];
     uint32_t st_new = st_old;
     float xh4[4], rb4[4], j4[4];
     int zcheck4 = 0;
