@@ -1005,25 +1005,33 @@ struct TsRun {
         MRX_TRY(ensure_buffers(ctx, N, 1));
         MRX_TRY(ensure_buf(ctx, ctx->ts_work, ctx->ts_work_cap, np));
         MRX_TRY(ensure_buf(ctx, ctx->ts_shells, ctx->ts_shells_cap, (size_t)s->nsteps * ncell));
-        HIP_TRY(hipMemcpyAsync(ctx->ts_work, sfr, sizeof(float) * np, hipMemcpyDeviceToDevice, ctx->stream));
         // f_coll and <x_e> of the first shell (ComputeTs.c:413-475) read the
         // input grids only: taken first, so the host builds the tables while
         // the device filters the shells
         if (!p->fixed_globals) MRX_TRY(fcoll(s->R[0], 1));
         const long half = (long)N * N * (N / 2 + 1);
+        // the working grid starts as the input sfr grid: it is read in place
+        // until the first shell with operations writes ts_work from it
+        const float* wgrid = sfr;
         for (int ct = 0; ct < s->nsteps; ++ct) {
-          for (int k = 0; k < s->n_ops[ct]; ++k) {
-            const mrx_sfr_op& o = s->ops[ct][k];
-            const long n = (o.kind <= 1) ? half : ncell;
-            const float* hs = (o.kind >= 2) ? hist + (size_t)o.slot * np : nullptr;
+          if (s->n_ops[ct] > 0) {
+            SfrOps o;
+            o.n = s->n_ops[ct];
+            for (int k = 0; k < o.n; ++k) {
+              o.kind[k] = s->ops[ct][k].kind;
+              o.slot[k] = s->ops[ct][k].slot;
+              o.wd[k] = s->ops[ct][k].weight;
+              o.wf[k] = (float)s->ops[ct][k].weight;
+            }
             PROF("sfr_op");
-            k_sfr_op<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(ctx->ts_work, hs, o.kind, o.weight,
-                                                                          (float)o.weight, N, half);
-            MRX_TRY(check_launch(ctx, "k_sfr_op"));
+            k_sfr_ops<<<(unsigned)((np + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->ts_work, wgrid, hist, (long)np,
+                                                                              o, N, half);
+            MRX_TRY(check_launch(ctx, "k_sfr_ops"));
+            wgrid = ctx->ts_work;
           }
           // the working grid only changes when operations were applied: the
           // forward transform of an unchanged grid is reused
-          if (ct == 0 || s->n_ops[ct] > 0) MRX_TRY(P::forward(ctx, ctx->ts_work, 0, 1, true, ctx->U, ctx->V, true));
+          if (ct == 0 || s->n_ops[ct] > 0) MRX_TRY(P::forward(ctx, wgrid, 0, 1, true, ctx->U, ctx->V, true));
           const WinArgs w = win_args(p->box_size, (float)s->R[ct], p->heating_filter_type, ct > 0 ? 1 : 0);
           int kzlive;
           MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w, kzlive));

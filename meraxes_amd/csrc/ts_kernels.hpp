@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "reion_kernels.hpp"
+#include "ts_tables.h"
 
 namespace mrx {
 
@@ -476,6 +477,42 @@ __global__ void k_sfr_op(float* __restrict__ W, const float* __restrict__ hist, 
   const float v = hist[i] * wf;
   const long ip = r * (N + 2) + z;
   W[ip] = (kind == 2) ? v : W[ip] + v;
+}
+
+// The operations of one shell (ComputeTs.c:276-377, in plan order) in one
+// pass: W[ip] = the operations applied in order to src[ip], every padded
+// element written (src may be W, or the input grid the working grid starts
+// as).  Each element sees k_sfr_op's float operations in the same order.
+struct SfrOps {
+  int n;
+  int kind[MRX_TS_MAX_OPS];
+  int slot[MRX_TS_MAX_OPS];
+  double wd[MRX_TS_MAX_OPS];
+  float wf[MRX_TS_MAX_OPS];
+};
+
+__global__ __launch_bounds__(kNT) void k_sfr_ops(float* __restrict__ W, const float* src,
+                                                const float* __restrict__ hist, long hstride, SfrOps o, int N,
+                                                long half)
+{
+  const long np = (long)N * N * (N + 2);
+  const long ip = (long)blockIdx.x * kNT + threadIdx.x;
+  if (ip >= np) return;
+  const long r = ip / (N + 2);
+  const int z = (int)(ip % (N + 2));
+  const bool real = z < N;
+  const long i = r * N + z;
+  float v = src[ip];
+  for (int k = 0; k < o.n; ++k) {
+    const int kind = o.kind[k];
+    if (kind <= 1) {
+      if (ip < half) v = (kind == 0) ? 0.0f : (float)((double)v * o.wd[k]);
+    } else if (real) {
+      const float h = hist[(long)o.slot[k] * hstride + i] * o.wf[k];
+      v = (kind == 2) ? h : v + h;
+    }
+  }
+  W[ip] = v;
 }
 
 #pragma clang fp contract(on)
