@@ -420,6 +420,39 @@ struct Pipe {
     return check_launch(ctx, "k_unpack_dc");
   }
 
+  // forward() for one field whose spectrum a sharp-k window will cut to
+  // |n_x|, |n_y|, n_z <= nmax (nmax < N/2): the y pass over the kz blocks
+  // that hold kz <= nmax, the x pass over those blocks of the lines |ky| <=
+  // nmax; the other tiles of U (and the Nyquist plane, which the window
+  // zeroes) are left stale and must not be read.  The tiles it computes are
+  // forward()'s, bit for bit.
+  static int forward_sharpk(mrx_ctx* ctx, const float* in, float2* U, float2* V, int nmax)
+  {
+    constexpr int TW = Cfg<N>::TW;
+    const int nkzb = std::min(NH / TW, nmax / TW + 1);
+    {
+      PROF("r2c_z");
+      k_r2c_z<N><<<dim3(N * N / Cfg<N>::RZ, 1), kNT, 0, ctx->stream>>>(in, U, ctx->tw, 0, nc);
+    }
+    MRX_TRY(check_launch(ctx, "k_r2c_z"));
+    {
+      PROF("colpass_y_fwd");
+      k_colpass_sub<N, false><<<dim3(N * nkzb, 1), Cfg<N>::COL_NT, 0, ctx->stream>>>(U, ctx->tw, nc, (long)N * NH,
+                                                                                     NH, 0.0, nkzb, -1);
+    }
+    MRX_TRY(check_launch(ctx, "k_colpass_sub y"));
+    const int nline = std::min(N, 2 * nmax + 1);
+    {
+      PROF("colpass_x_fwd");
+      k_colpass_sub<N, false><<<dim3(nline * nkzb, 1), Cfg<N>::COL_NT, 0, ctx->stream>>>(
+        U, ctx->tw, nc, NH, (long)N * NH, -std::pow((double)N, 3), nkzb, nmax);
+    }
+    MRX_TRY(check_launch(ctx, "k_colpass_sub x"));
+    PROF("unpack_dc");
+    k_unpack_dc<N><<<dim3((N * N + 255) / 256, 1), 256, 0, ctx->stream>>>(U, V, nc, (long)N * N);
+    return check_launch(ctx, "k_unpack_dc");
+  }
+
   // U (+V) filtered at R -> T, then inverse y pass; T then holds the packed
   // spectra of the filtered fields with x and y already transformed, in kz
   // slots below kzlive (the slots above are zero and left unwritten: a
@@ -1030,9 +1063,18 @@ struct TsRun {
             wgrid = ctx->ts_work;
           }
           // the working grid only changes when operations were applied: the
-          // forward transform of an unchanged grid is reused
-          if (ct == 0 || s->n_ops[ct] > 0) MRX_TRY(P::forward(ctx, wgrid, 0, 1, true, ctx->U, ctx->V, true));
+          // forward transform of an unchanged grid is reused.  With sharp-k
+          // shells, the shells that reuse it have radii >= this one's, so
+          // their balls of modes lie inside this shell's: the forward passes
+          // need only its tiles (ct = 0 is unfiltered and takes them all).
           const WinArgs w = win_args(p->box_size, (float)s->R[ct], p->heating_filter_type, ct > 0 ? 1 : 0);
+          if (ct == 0 || s->n_ops[ct] > 0) {
+            const int nmax = (w.apply && w.type == MRX_FILTER_SHARPK) ? sharpk_nmax(w, N) : N / 2;
+            if (nmax < N / 2)
+              MRX_TRY(P::forward_sharpk(ctx, wgrid, ctx->U, ctx->V, nmax));
+            else
+              MRX_TRY(P::forward(ctx, wgrid, 0, 1, true, ctx->U, ctx->V, true));
+          }
           int kzlive;
           MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w, kzlive));
           MRX_TRY(P::c2r_z(ctx, ctx->T, ctx->ts_shells + (size_t)ct * ncell, 1, 3, N, kzlive));

@@ -354,6 +354,55 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass(float2* __r
   for (int i = 0; i < EPT; ++i) g[(unsigned)(i * RPI) * rs] = r[i];
 }
 
+// k_colpass over a subset of its tiles: kz blocks below nkzb only (grid
+// nline x nkzb per field, tile t = line t / nkzb, kz block t % nkzb), and
+// with lmax >= 0 only the lines ky with min(ky, N - ky) <= lmax (line j is ky
+// = j for j <= lmax, else N - (nline - j), nline = min(N, 2 lmax + 1)).  Each
+// tile is transformed exactly as by k_colpass; the others are left as they
+// are.  The forward passes of a spectrum that a sharp-k window will cut to
+// the ball of modes |n_x|, |n_y|, n_z <= nmax need only these tiles.
+template <int N, bool INV, int TW = Cfg<N>::TW>
+__global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass_sub(float2* __restrict__ data,
+                                                                   const float2* __restrict__ twN, long fstride,
+                                                                   long astride, long rstride, double div, int nkzb,
+                                                                   int lmax)
+{
+  using M = TileMap<N, TW>;
+  constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT;
+  __shared__ float2 s[N * SP];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  const int c = tid % TW, n0 = tid / TW;
+  const int j = blockIdx.x / nkzb, b = blockIdx.x % nkzb;
+  int a = j;
+  if (lmax >= 0) {
+    const int nline = min(N, 2 * lmax + 1);
+    a = (j <= lmax) ? j : N - (nline - j);
+  }
+  const unsigned rs = (unsigned)rstride;
+  float2* g = data + blockIdx.y * fstride + (long)a * astride + (long)b * TW + (unsigned)n0 * rs + c;
+  float2 r[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) r[i] = g[(unsigned)(i * RPI) * rs];
+  for (int i = tid; i < N; i += NT) tw[i] = twN[i];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = r[i];
+  __syncthreads();
+  fft_lds<N, TW, NT, INV, SP, 1, true, 1, false, 0>(s, tw, tid);
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) r[i] = s[(n0 + i * RPI) * SP + c];
+  if (div > 0.0) {
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) r[i] = make_float2((float)((double)r[i].x / div), (float)((double)r[i].y / div));
+  } else if (div < 0.0) {
+    const float fd = (float)(-div);
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) r[i] = make_float2(r[i].x / fd, r[i].y / fd);
+  }
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) g[(unsigned)(i * RPI) * rs] = r[i];
+}
+
 // Window table of one radius: W depends on (|nx|, |ny|, nz) only, because
 // filter() builds each k component as n*dk or (n-N)*dk and squares it, so
 // modes n and N-n give the same float k^2.  Layout for tile width TW (kz
