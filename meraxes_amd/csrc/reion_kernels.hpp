@@ -292,6 +292,22 @@ __global__ __launch_bounds__(kNT) void k_c2r_z(const float2* __restrict__ in, fl
   const long row0 = (long)blockIdx.x * RZ;
   c2r_rows_lds<N>(s, tw, in + blockIdx.y * in_fstride, row0, tid, kzlive);
   float* o = out + blockIdx.y * out_fstride;
+  if (pitch == N) {
+    // unpadded rows: each thread stores groups of 4 cells (two complex slots
+    // of one 32-slot LDS bank row) as one 16-byte write
+    constexpr int Q = N / 4;
+    static_assert((RZ * Q) % kNT == 0, "4-cell groups must split evenly over the threads");
+    float4* o4 = reinterpret_cast<float4*>(o + row0 * N);
+#pragma unroll
+    for (int i = 0; i < RZ * Q / kNT; ++i) {
+      const int q = tid + i * kNT;
+      const int c0 = (q / Q) * (N / 2) + 2 * (q % Q);
+      const float2 a = s[lds_addr<true>(c0)], b = s[lds_addr<true>(c0 + 1)];
+      o4[q] = make_float4(clamp_kind(a.x, clamp), clamp_kind(a.y, clamp), clamp_kind(b.x, clamp),
+                          clamp_kind(b.y, clamp));
+    }
+    return;
+  }
   for (int e = tid; e < RZ * pitch; e += kNT) {
     const int r = e / pitch, z = e % pitch;
     o[(row0 + r) * pitch + z] = (z < N) ? clamp_kind(lds_real(s, r * N + z), clamp) : 0.0f;

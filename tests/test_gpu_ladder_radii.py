@@ -1,7 +1,7 @@
 """The bubble ladder's multi-radius x pass is the single-radius one, bit for bit.
 
 With tabulated windows (top-hat, Gaussian) the inverse x pass covers several
-radii per launch (MRX_LADDER_RADII, read when a context is created, 4 by
+radii per launch (MRX_LADDER_RADII, read when a context is created, 2 by
 default).  Each radius's spectra are those of the one-radius pass, so every
 output grid and global must be identical for 1, 3 (groups that do not divide
 the ladder) and 4 radii per pass.
