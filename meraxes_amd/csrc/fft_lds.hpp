@@ -27,6 +27,12 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// streaming store: the line is not kept in the caches for a later read
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(float2* p, float2 v)
+{
+  __builtin_nontemporal_store(f32x2{v.x, v.y}, reinterpret_cast<f32x2*>(p));
+}
 // multiply by (sign * i): INV=false -> -i (forward kernel exp(-i...)), INV=true -> +i
 template <bool INV>
 __device__ __forceinline__ float2 mul_si(float2 a)

@@ -1057,7 +1057,7 @@ struct TsRun {
               o.wf[k] = (float)s->ops[ct][k].weight;
             }
             PROF("sfr_op");
-            k_sfr_ops<<<(unsigned)((np + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->ts_work, wgrid, hist, (long)np,
+            k_sfr_ops<<<(unsigned)((np / 2 + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->ts_work, wgrid, hist, (long)np,
                                                                               o, N, half);
             MRX_TRY(check_launch(ctx, "k_sfr_ops"));
             wgrid = ctx->ts_work;

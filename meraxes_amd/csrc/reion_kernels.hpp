@@ -367,7 +367,7 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass(float2* __r
     for (int i = 0; i < EPT; ++i) r[i] = make_float2(r[i].x / fd, r[i].y / fd);
   }
 #pragma unroll
-  for (int i = 0; i < EPT; ++i) g[(unsigned)(i * RPI) * rs] = r[i];
+  for (int i = 0; i < EPT; ++i) store_nt(g + (unsigned)(i * RPI) * rs, r[i]);
 }
 
 // k_colpass over a subset of its tiles: kz blocks below nkzb only (grid
@@ -626,7 +626,7 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT)) __attribute__((amdgpu_waves_p
     fft_lds<N, TW, NT, true, SP, 1, true, 1, false, 0>(s, tw, tid);
     float2* t = T + k * tstride + f * fstride + base;
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) t[(unsigned)(i * RPI) * rs] = s[(n0 + i * RPI) * SP + c];
+    for (int i = 0; i < EPT; ++i) store_nt(t + (unsigned)(i * RPI) * rs, s[(n0 + i * RPI) * SP + c]);
     __syncthreads();
   }
 }

@@ -491,28 +491,35 @@ struct SfrOps {
   float wf[MRX_TS_MAX_OPS];
 };
 
+// Cells are taken in pairs (z, z+1), z even: a padded row of N + 2 floats
+// holds N/2 + 1 pairs, the last the padding, and `half` (a whole number of
+// rows' worth of floats, even) never splits a pair.
 __global__ __launch_bounds__(kNT) void k_sfr_ops(float* __restrict__ W, const float* src,
                                                 const float* __restrict__ hist, long hstride, SfrOps o, int N,
                                                 long half)
 {
-  const long np = (long)N * N * (N + 2);
-  const long ip = (long)blockIdx.x * kNT + threadIdx.x;
-  if (ip >= np) return;
-  const long r = ip / (N + 2);
-  const int z = (int)(ip % (N + 2));
-  const bool real = z < N;
-  const long i = r * N + z;
-  float v = src[ip];
+  const unsigned ph = (unsigned)(N / 2 + 1);
+  const unsigned npair = (unsigned)N * (unsigned)N * ph;
+  const unsigned p = blockIdx.x * kNT + threadIdx.x;
+  if (p >= npair) return;
+  const unsigned r = p / ph;
+  const unsigned z = 2 * (p - r * ph);
+  const bool real = z < (unsigned)N;
+  const bool lo = 2 * (long)p < half;
+  const long i = (long)r * N + z;
+  float2 v = reinterpret_cast<const float2*>(src)[p];
   for (int k = 0; k < o.n; ++k) {
     const int kind = o.kind[k];
     if (kind <= 1) {
-      if (ip < half) v = (kind == 0) ? 0.0f : (float)((double)v * o.wd[k]);
+      if (lo) v = (kind == 0) ? make_float2(0.0f, 0.0f)
+                              : make_float2((float)((double)v.x * o.wd[k]), (float)((double)v.y * o.wd[k]));
     } else if (real) {
-      const float h = hist[(long)o.slot[k] * hstride + i] * o.wf[k];
-      v = (kind == 2) ? h : v + h;
+      const float2 h = *reinterpret_cast<const float2*>(hist + (long)o.slot[k] * hstride + i);
+      const float2 hw = make_float2(h.x * o.wf[k], h.y * o.wf[k]);
+      v = (kind == 2) ? hw : make_float2(v.x + hw.x, v.y + hw.y);
     }
   }
-  W[ip] = v;
+  reinterpret_cast<float2*>(W)[p] = v;
 }
 
 #pragma clang fp contract(on)
