@@ -20,6 +20,7 @@
 #include <float.h>
 #include <math.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -414,22 +415,29 @@ int mrx_ts_locate_xHII_index(const mrx_ts_data* d, float xHII_call)
   return m;
 }
 
-/* bilinear interpolation of one x_int table in float, as the interp_*
- * functions do (XRayHeatingFunctions.c:914-1192); `below` is returned under
- * the lowest tabulated energy (1 for f_heat, 0 otherwise) */
-float mrx_ts_interp_xint(const mrx_ts_data* d, const float (*tab)[MRX_X_INT_NENERGY], float En, float xHII_call,
-                         float below)
+/* The energy side of the x_int interpolation: the clamped energy and its bin,
+ * or -1 under the lowest tabulated energy (XRayHeatingFunctions.c:914-1192). */
+static int xint_energy_bin(const mrx_ts_data* d, float* En)
 {
   const float* E = d->xi_E;
-  if (En > 0.999 * E[MRX_X_INT_NENERGY - 1])
-    En = (float)(E[MRX_X_INT_NENERGY - 1] * 0.999);
-  else if (En < E[0])
-    return below;
+  if (*En > 0.999 * E[MRX_X_INT_NENERGY - 1])
+    *En = (float)(E[MRX_X_INT_NENERGY - 1] * 0.999);
+  else if (*En < E[0])
+    return -1;
+  return locate_energy_index(*En);
+}
+
+/* The x_e side: bilinear interpolation in float at a clamped energy En in bin
+ * n_low, as the interp_* functions do. */
+static float xint_at(const mrx_ts_data* d, const float (*tab)[MRX_X_INT_NENERGY], float En, int n_low,
+                     float xHII_call)
+{
+  const float* E = d->xi_E;
   if (xHII_call > d->xi_XHII[MRX_X_INT_NXHII - 1] * 0.999)
     xHII_call = (float)(d->xi_XHII[MRX_X_INT_NXHII - 1] * 0.999);
   else if (xHII_call < d->xi_XHII[0])
     xHII_call = (float)(1.001 * d->xi_XHII[0]);
-  const int n_low = locate_energy_index(En), n_high = n_low + 1;
+  const int n_high = n_low + 1;
   const int m_low = mrx_ts_locate_xHII_index(d, xHII_call), m_high = m_low + 1;
   float elow = ((tab[m_low][n_high] - tab[m_low][n_low]) / (E[n_high] - E[n_low]));
   elow *= (En - E[n_low]);
@@ -441,6 +449,16 @@ float mrx_ts_interp_xint(const mrx_ts_data* d, const float (*tab)[MRX_X_INT_NENE
   r *= (xHII_call - d->xi_XHII[m_low]);
   r += elow;
   return r;
+}
+
+/* bilinear interpolation of one x_int table in float, as the interp_*
+ * functions do (XRayHeatingFunctions.c:914-1192); `below` is returned under
+ * the lowest tabulated energy (1 for f_heat, 0 otherwise) */
+float mrx_ts_interp_xint(const mrx_ts_data* d, const float (*tab)[MRX_X_INT_NENERGY], float En, float xHII_call,
+                         float below)
+{
+  const int n_low = xint_energy_bin(d, &En);
+  return (n_low < 0) ? below : xint_at(d, tab, En, n_low, xHII_call);
 }
 
 /* ---------------------------------------------------- cross sections */
@@ -728,65 +746,148 @@ double mrx_ts_nu_tau_one(const mrx_ts_data* d, const mrx_ts_params* p, double zp
 
 /* ------------------------------------------------- frequency integrals */
 
+/* What the frequency integrands need at a node nu that does not depend on
+ * x_e: the HI, HeI and HeII cross sections, the spectrum factor, and per
+ * species threshold the interpolation energy and its bin (-1: below). */
+typedef struct {
+  double nu;
+  double xs[3], pw;
+  float En[3];
+  int bin[3];
+} nu_node;
+
+static const double k_nu_th[3] = {K_NUIONIZATION, K_HeI_NUIONIZATION, K_HeII_NUIONIZATION};
+
+static void nu_node_eval(const mrx_ts_data* d, double thresh, double spec_index, double nu, nu_node* v)
+{
+  v->nu = nu;
+  v->xs[0] = HI_ion_crosssec(nu);
+  v->xs[1] = HeI_ion_crosssec(nu);
+  v->xs[2] = HeII_ion_crosssec(nu);
+  v->pw = pow(nu / (thresh * K_NU_over_EV), -spec_index - 1);
+  for (int i = 0; i < 3; i++) {
+    v->En[i] = (float)((nu - k_nu_th[i]) / K_NU_over_EV);
+    v->bin[i] = xint_energy_bin(d, &v->En[i]);
+  }
+}
+
+/* The integrals of one shell share their interval [lower, nu_max], and QAG
+ * bisects it the same way for every x_e, so they evaluate mostly the same
+ * nodes: the node values are kept in an open-addressing table keyed by the
+ * bits of nu (the same doubles as evaluating them again). */
+enum { NU_CACHE_BITS = 14, NU_CACHE_SIZE = 1 << NU_CACHE_BITS, NU_CACHE_FILL = NU_CACHE_SIZE / 2 };
+typedef struct {
+  int count;
+  unsigned char used[NU_CACHE_SIZE];
+  nu_node e[NU_CACHE_SIZE];
+} nu_cache;
+
+static void nu_cache_clear(nu_cache* c)
+{
+  c->count = 0;
+  memset(c->used, 0, sizeof c->used);
+}
+
 typedef struct {
   const mrx_ts_data* d;
   double x_e, thresh, spec_index;
+  nu_cache* cache; /* NULL: every node evaluated */
+  nu_node scratch;
 } nu_par;
+
+static const nu_node* node_at(nu_par* q, double nu)
+{
+  nu_cache* c = q->cache;
+  if (!c) {
+    nu_node_eval(q->d, q->thresh, q->spec_index, nu, &q->scratch);
+    return &q->scratch;
+  }
+  uint64_t key;
+  memcpy(&key, &nu, sizeof key);
+  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> (64 - NU_CACHE_BITS));
+  for (;;) {
+    if (!c->used[h]) break;
+    if (memcmp(&c->e[h].nu, &nu, sizeof nu) == 0) return &c->e[h];
+    h = (h + 1) & (NU_CACHE_SIZE - 1);
+  }
+  if (c->count >= NU_CACHE_FILL) {
+    nu_cache_clear(c);
+    h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> (64 - NU_CACHE_BITS));
+  }
+  c->used[h] = 1;
+  c->count++;
+  nu_node_eval(q->d, q->thresh, q->spec_index, nu, &c->e[h]);
+  return &c->e[h];
+}
+
+static float xint_node(const mrx_ts_data* d, const float (*tab)[MRX_X_INT_NENERGY], const nu_node* v, int i,
+                       float xe, float below)
+{
+  return (v->bin[i] < 0) ? below : xint_at(d, tab, v->En[i], v->bin[i], xe);
+}
 
 static double nu_heat(double nu, void* vp)
 {
-  const nu_par* q = (const nu_par*)vp;
+  nu_par* q = (nu_par*)vp;
   const mrx_ts_data* d = q->d;
-  double s = mrx_ts_interp_xint(d, d->xi_fheat, (float)((nu - K_NUIONIZATION) / K_NU_over_EV), (float)q->x_e, 1.0f) *
-             K_PLANCK * (nu - K_NUIONIZATION) * d->f_H * (1 - q->x_e) * HI_ion_crosssec(nu);
-  s += mrx_ts_interp_xint(d, d->xi_fheat, (float)((nu - K_HeI_NUIONIZATION) / K_NU_over_EV), (float)q->x_e, 1.0f) *
-       K_PLANCK * (nu - K_HeI_NUIONIZATION) * d->f_He * (1 - q->x_e) * HeI_ion_crosssec(nu);
-  s += mrx_ts_interp_xint(d, d->xi_fheat, (float)((nu - K_HeII_NUIONIZATION) / K_NU_over_EV), (float)q->x_e, 1.0f) *
-       K_PLANCK * (nu - K_HeII_NUIONIZATION) * d->f_He * q->x_e * HeII_ion_crosssec(nu);
-  return s * pow(nu / (q->thresh * K_NU_over_EV), -q->spec_index - 1);
+  const nu_node* v = node_at(q, nu);
+  const float xe = (float)q->x_e;
+  double s = xint_node(d, d->xi_fheat, v, 0, xe, 1.0f) * K_PLANCK * (nu - K_NUIONIZATION) * d->f_H * (1 - q->x_e) *
+             v->xs[0];
+  s += xint_node(d, d->xi_fheat, v, 1, xe, 1.0f) * K_PLANCK * (nu - K_HeI_NUIONIZATION) * d->f_He * (1 - q->x_e) *
+       v->xs[1];
+  s += xint_node(d, d->xi_fheat, v, 2, xe, 1.0f) * K_PLANCK * (nu - K_HeII_NUIONIZATION) * d->f_He * q->x_e *
+       v->xs[2];
+  return s * v->pw;
 }
 
-static double nion_sum(const mrx_ts_data* d, double nu, double nu_th, double x_e)
+static double nion_sum(const mrx_ts_data* d, const nu_node* v, int i, float xe)
 {
-  const float En = (float)((nu - nu_th) / K_NU_over_EV), xe = (float)x_e;
-  return mrx_ts_interp_xint(d, d->xi_nHI, En, xe, 0.0f) + mrx_ts_interp_xint(d, d->xi_nHeI, En, xe, 0.0f) +
-         mrx_ts_interp_xint(d, d->xi_nHeII, En, xe, 0.0f) + 1;
+  return xint_node(d, d->xi_nHI, v, i, xe, 0.0f) + xint_node(d, d->xi_nHeI, v, i, xe, 0.0f) +
+         xint_node(d, d->xi_nHeII, v, i, xe, 0.0f) + 1;
 }
 
 static double nu_ion(double nu, void* vp)
 {
-  const nu_par* q = (const nu_par*)vp;
+  nu_par* q = (nu_par*)vp;
   const mrx_ts_data* d = q->d;
-  double F_i = nion_sum(d, nu, K_NUIONIZATION, q->x_e);
-  double s = F_i * d->f_H * (1 - q->x_e) * HI_ion_crosssec(nu);
-  F_i = nion_sum(d, nu, K_HeI_NUIONIZATION, q->x_e);
-  s += F_i * d->f_He * (1 - q->x_e) * HeI_ion_crosssec(nu);
-  F_i = nion_sum(d, nu, K_HeII_NUIONIZATION, q->x_e);
-  s += F_i * d->f_He * q->x_e * HeII_ion_crosssec(nu);
-  return s * pow(nu / (q->thresh * K_NU_over_EV), -q->spec_index - 1);
+  const nu_node* v = node_at(q, nu);
+  const float xe = (float)q->x_e;
+  double F_i = nion_sum(d, v, 0, xe);
+  double s = F_i * d->f_H * (1 - q->x_e) * v->xs[0];
+  F_i = nion_sum(d, v, 1, xe);
+  s += F_i * d->f_He * (1 - q->x_e) * v->xs[1];
+  F_i = nion_sum(d, v, 2, xe);
+  s += F_i * d->f_He * q->x_e * v->xs[2];
+  return s * v->pw;
 }
 
 static double nu_lya(double nu, void* vp)
 {
-  const nu_par* q = (const nu_par*)vp;
+  nu_par* q = (nu_par*)vp;
   const mrx_ts_data* d = q->d;
-  double s = mrx_ts_interp_xint(d, d->xi_nLya, (float)((nu - K_NUIONIZATION) / K_NU_over_EV), (float)q->x_e, 0.0f) *
-             d->f_H * (1 - q->x_e) * HI_ion_crosssec(nu);
-  s += mrx_ts_interp_xint(d, d->xi_nLya, (float)((nu - K_HeI_NUIONIZATION) / K_NU_over_EV), (float)q->x_e, 0.0f) *
-       d->f_He * (1 - q->x_e) * HeI_ion_crosssec(nu);
-  s += mrx_ts_interp_xint(d, d->xi_nLya, (float)((nu - K_HeII_NUIONIZATION) / K_NU_over_EV), (float)q->x_e, 0.0f) *
-       d->f_He * q->x_e * HeII_ion_crosssec(nu);
-  return s * pow(nu / (q->thresh * K_NU_over_EV), -q->spec_index - 1);
+  const nu_node* v = node_at(q, nu);
+  const float xe = (float)q->x_e;
+  double s = xint_node(d, d->xi_nLya, v, 0, xe, 0.0f) * d->f_H * (1 - q->x_e) * v->xs[0];
+  s += xint_node(d, d->xi_nLya, v, 1, xe, 0.0f) * d->f_He * (1 - q->x_e) * v->xs[1];
+  s += xint_node(d, d->xi_nLya, v, 2, xe, 0.0f) * d->f_He * q->x_e * v->xs[2];
+  return s * v->pw;
+}
+
+static double integrate_over_nu(const mrx_ts_data* d, const mrx_ts_params* p, double zp, double local_x_e,
+                                double lower_int_limit, int flag, nu_cache* cache)
+{
+  nu_par q = {d, local_x_e, p->nu_xray_gal_threshold, p->spec_index_xray_gal, cache, {0}};
+  double (*f)(double, void*) = (flag == 0) ? nu_heat : (flag == 1 ? nu_ion : nu_lya);
+  const double result = mrx_ts_qag61(f, &q, lower_int_limit, p->nu_xray_max * K_NU_over_EV, 0.01, 1000, NULL);
+  if (flag == 2) return result * K_SPEED_OF_LIGHT / (4 * M_PI) / K_Ly_alpha_HZ / mrx_ts_hubble(d, (float)zp);
+  return result;
 }
 
 double mrx_ts_integrate_over_nu(const mrx_ts_data* d, const mrx_ts_params* p, double zp, double local_x_e,
                                 double lower_int_limit, int flag)
 {
-  nu_par q = {d, local_x_e, p->nu_xray_gal_threshold, p->spec_index_xray_gal};
-  double (*f)(double, void*) = (flag == 0) ? nu_heat : (flag == 1 ? nu_ion : nu_lya);
-  const double result = mrx_ts_qag61(f, &q, lower_int_limit, p->nu_xray_max * K_NU_over_EV, 0.01, 1000, NULL);
-  if (flag == 2) return result * K_SPEED_OF_LIGHT / (4 * M_PI) / K_Ly_alpha_HZ / mrx_ts_hubble(d, (float)zp);
-  return result;
+  return integrate_over_nu(d, p, zp, local_x_e, lower_int_limit, flag, NULL);
 }
 
 /* -------------------------------------------------- per-snapshot plan */
@@ -907,7 +1008,8 @@ int mrx_ts_plan(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p)
 
 /* The shells' tables are independent of one another: phase 0 computes each
  * shell's lower frequency limit and raw Ly-n sum, phase 1 the 3 x 14
- * frequency integrals of every shell; work items are handed out one at a
+ * frequency integrals of each shell (one work item per shell, its integrals
+ * sharing one table of node values); work items are handed out one at a
  * time to up to 16 host threads.  Each item is computed exactly as the serial
  * loop would, so the tables do not depend on the thread count. */
 typedef struct {
@@ -919,7 +1021,7 @@ typedef struct {
   int next; /* next unclaimed work item */
 } tab_job;
 
-static void tab_item(tab_job* j, int it)
+static void tab_item(tab_job* j, int it, nu_cache* cache)
 {
   mrx_ts_snap* s = j->s;
   const mrx_ts_data* d = j->d;
@@ -939,25 +1041,28 @@ static void tab_item(tab_job* j, int it)
     }
     s->sum_lyn[ct] = sum;
   } else {
-    const int ct = it / (3 * MRX_X_INT_NXHII), x = (it / 3) % MRX_X_INT_NXHII, kind = it % 3;
-    const double v = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], kind);
-    if (kind == 0)
-      s->freq_heat[x][ct] = v;
-    else if (kind == 1)
-      s->freq_ion[x][ct] = v;
-    else
-      s->freq_lya[x][ct] = v;
+    const int ct = it;
+    if (cache) nu_cache_clear(cache);
+    for (int x = 0; x < MRX_X_INT_NXHII; x++) {
+      s->freq_heat[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], 0, cache);
+      s->freq_ion[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], 1, cache);
+      s->freq_lya[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], 2, cache);
+    }
   }
 }
 
 static void* tab_worker(void* vp)
 {
   tab_job* j = (tab_job*)vp;
+  /* without the table (allocation failed) every node is evaluated: the same values */
+  nu_cache* cache = (j->phase == 1) ? (nu_cache*)malloc(sizeof(nu_cache)) : NULL;
   for (;;) {
     const int it = __atomic_fetch_add(&j->next, 1, __ATOMIC_RELAXED);
-    if (it >= j->nitems) return NULL;
-    tab_item(j, it);
+    if (it >= j->nitems) break;
+    tab_item(j, it, cache);
   }
+  free(cache);
+  return NULL;
 }
 
 static void tab_run(tab_job* j)
@@ -988,7 +1093,7 @@ int mrx_ts_build_tables(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_param
   tab_job job = {s, d, p, lower, 0, s->nsteps, 0};
   tab_run(&job);
   job.phase = 1;
-  job.nitems = s->nsteps * MRX_X_INT_NXHII * 3;
+  job.nitems = s->nsteps;
   job.next = 0;
   tab_run(&job);
   for (int ct = 0; ct < s->nsteps; ct++) {
