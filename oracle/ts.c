@@ -536,3 +536,43 @@ int orc_ts_export(const mrx_ts_params* p, double* out, int cap)
   free(d);
   return rc == MRX_OK ? n : rc;
 }
+
+/* Builds the snapshot's tables (threaded, each shell's integrals sharing node
+ * values) and recomputes every frequency integral alone with
+ * mrx_ts_integrate_over_nu (no shared values, one thread), with the shell's
+ * lower limit formed as ComputeTs.c does; returns how many of the
+ * 3 x 14 x nsteps entries differ in any bit, or a negative error. */
+int orc_ts_tables_recheck(const mrx_ts_params* p, double collapse_fraction, double x_e_ave)
+{
+  const double nu_over_ev = 1.60217646e-12 / 6.6262e-27; /* eV -> Hz, the tables' constant */
+  mrx_ts_data* d = (mrx_ts_data*)orc_ts_data_new(p);
+  if (!d) return MRX_EINVAL;
+  mrx_ts_snap* s = (mrx_ts_snap*)calloc(1, sizeof(mrx_ts_snap));
+  int rc = s ? mrx_ts_plan(s, d, p) : MRX_ENOMEM;
+  if (rc == MRX_OK && s->homogeneous) rc = MRX_EUNSUPPORTED;
+  if (rc == MRX_OK) {
+    s->collapse_fraction = collapse_fraction;
+    s->x_e_ave = x_e_ave;
+    rc = mrx_ts_build_tables(s, d, p);
+  }
+  int bad = 0;
+  if (rc == MRX_OK) {
+    for (int ct = 0; ct < s->nsteps; ct++) {
+      const double zp = s->zp, prev = (ct == 0) ? zp : s->zpp_edge[ct - 1];
+      const double zpp = (s->zpp_edge[ct] + prev) * 0.5;
+      const double lower = fmax(mrx_ts_nu_tau_one(d, p, zp, zpp, x_e_ave, x_e_ave),
+                                p->nu_xray_gal_threshold * nu_over_ev);
+      for (int x = 0; x < MRX_X_INT_NXHII; x++) {
+        const double h = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 0);
+        const double i = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 1);
+        const double l = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 2);
+        bad += memcmp(&h, &s->freq_heat[x][ct], sizeof h) != 0;
+        bad += memcmp(&i, &s->freq_ion[x][ct], sizeof i) != 0;
+        bad += memcmp(&l, &s->freq_lya[x][ct], sizeof l) != 0;
+      }
+    }
+  }
+  free(s);
+  free(d);
+  return rc == MRX_OK ? bad : rc;
+}

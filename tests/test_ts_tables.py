@@ -68,6 +68,21 @@ def test_qag61_matches_quad(L):
     assert abs(got - ref) <= 0.01 * ref
 
 
+
+@pytest.mark.parametrize("snap,cf,xe", [(24, 3e-3, 2e-3), (12, 1e-2, 0.05)])
+def test_frequency_tables_equal_integrals_taken_alone(L, snap, cf, xe):
+    """The threaded table build, whose 42 integrals per shell share their
+    x_e-independent node values, equals every integral recomputed alone, in
+    every bit."""
+    L.orc_ts_tables_recheck.restype = C.c_int
+    L.orc_ts_tables_recheck.argtypes = [C.POINTER(P.TsParams), C.c_double, C.c_double]
+    z = P.snapshot_redshifts()
+    fc = np.zeros(len(z))
+    fc[:snap] = np.linspace(1e-4, 5e-3, snap)
+    p = P.ts_params(64, 100.0, snap, z, fc, 8)
+    assert L.orc_ts_tables_recheck(C.byref(p), cf, xe) == 0
+
+
 def test_brent_matches_brentq(L):
     f = FN(lambda x, _: math.cos(x) - x)
     got = L.orc_brent(f, 0.0, 1.0, 1e-12)
