@@ -1006,48 +1006,41 @@ int mrx_ts_plan(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p)
   return MRX_OK;
 }
 
-/* The shells' tables are independent of one another: phase 0 computes each
- * shell's lower frequency limit and raw Ly-n sum, phase 1 the 3 x 14
- * frequency integrals of each shell (one work item per shell, its integrals
- * sharing one table of node values); work items are handed out one at a
- * time to up to 16 host threads.  Each item is computed exactly as the serial
- * loop would, so the tables do not depend on the thread count. */
+/* The shells' tables are independent of one another: one work item is a
+ * shell, its lower frequency limit, raw Ly-n sum and 3 x 14 frequency
+ * integrals (which share one table of node values); items are handed out one
+ * at a time to up to 16 host threads.  Each item is computed exactly as the
+ * serial loop would, so the tables do not depend on the thread count. */
 typedef struct {
   mrx_ts_snap* s;
   const mrx_ts_data* d;
   const mrx_ts_params* p;
-  double* lower;
-  int phase, nitems;
+  int nitems;
   int next; /* next unclaimed work item */
 } tab_job;
 
-static void tab_item(tab_job* j, int it, nu_cache* cache)
+static void tab_item(tab_job* j, int ct, nu_cache* cache)
 {
   mrx_ts_snap* s = j->s;
   const mrx_ts_data* d = j->d;
   const mrx_ts_params* p = j->p;
   const double zp = s->zp;
-  if (j->phase == 0) {
-    const int ct = it;
-    const double prev_zpp = (ct == 0) ? zp : s->zpp_edge[ct - 1];
-    const double zpp = (s->zpp_edge[ct] + prev_zpp) * 0.5;
-    j->lower[ct] = fmax(mrx_ts_nu_tau_one(d, p, zp, zpp, s->x_e_ave, s->x_e_ave),
-                        p->nu_xray_gal_threshold * K_NU_over_EV);
-    double sum = 0;
-    for (int n_ct = MRX_NSPEC_MAX; n_ct >= 2; n_ct--) {
-      if (zpp > zmax((float)zp, n_ct)) continue;
-      const double nuprime = nu_n(n_ct) * (1 + zpp) / (1.0 + zp);
-      sum += frecycle(n_ct) * mrx_ts_spectral_emissivity(d, nuprime);
-    }
-    s->sum_lyn[ct] = sum;
-  } else {
-    const int ct = it;
-    if (cache) nu_cache_clear(cache);
-    for (int x = 0; x < MRX_X_INT_NXHII; x++) {
-      s->freq_heat[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], 0, cache);
-      s->freq_ion[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], 1, cache);
-      s->freq_lya[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], j->lower[ct], 2, cache);
-    }
+  const double prev_zpp = (ct == 0) ? zp : s->zpp_edge[ct - 1];
+  const double zpp = (s->zpp_edge[ct] + prev_zpp) * 0.5;
+  const double lower =
+    fmax(mrx_ts_nu_tau_one(d, p, zp, zpp, s->x_e_ave, s->x_e_ave), p->nu_xray_gal_threshold * K_NU_over_EV);
+  double sum = 0;
+  for (int n_ct = MRX_NSPEC_MAX; n_ct >= 2; n_ct--) {
+    if (zpp > zmax((float)zp, n_ct)) continue;
+    const double nuprime = nu_n(n_ct) * (1 + zpp) / (1.0 + zp);
+    sum += frecycle(n_ct) * mrx_ts_spectral_emissivity(d, nuprime);
+  }
+  s->sum_lyn[ct] = sum;
+  if (cache) nu_cache_clear(cache);
+  for (int x = 0; x < MRX_X_INT_NXHII; x++) {
+    s->freq_heat[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 0, cache);
+    s->freq_ion[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 1, cache);
+    s->freq_lya[x][ct] = integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 2, cache);
   }
 }
 
@@ -1055,7 +1048,7 @@ static void* tab_worker(void* vp)
 {
   tab_job* j = (tab_job*)vp;
   /* without the table (allocation failed) every node is evaluated: the same values */
-  nu_cache* cache = (j->phase == 1) ? (nu_cache*)malloc(sizeof(nu_cache)) : NULL;
+  nu_cache* cache = (nu_cache*)malloc(sizeof(nu_cache));
   for (;;) {
     const int it = __atomic_fetch_add(&j->next, 1, __ATOMIC_RELAXED);
     if (it >= j->nitems) break;
@@ -1089,12 +1082,7 @@ int mrx_ts_build_tables(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_param
   int first_radii = 1, first_zero = 1;
   const int n_pts_radii = 1000;
   double weight = s->final_weight; /* the shell loop's last weight (one variable in the reference) */
-  double lower[MRX_TS_MAX_STEPS];
-  tab_job job = {s, d, p, lower, 0, s->nsteps, 0};
-  tab_run(&job);
-  job.phase = 1;
-  job.nitems = s->nsteps;
-  job.next = 0;
+  tab_job job = {s, d, p, s->nsteps, 0};
   tab_run(&job);
   for (int ct = 0; ct < s->nsteps; ct++) {
     double prev_zpp, prev_R;
