@@ -570,8 +570,10 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
 // k_window_table's layout), transforms it and writes it to T + k tstride.  Per
 // radius the result is k_ix_filter<N, kWinTable>'s, operation for operation.
 // A workgroup reads its tile of U once per radius, K times within a few
-// microseconds, so the repeats are served by the caches and HBM sees about
-// one read of U per K radii.
+// microseconds; at 512^3 with K = 2 the repeat still misses L2 (the L2
+// misses come to about two reads of U per launch), to be served by the
+// memory-side cache or HBM.  The outputs go out with streaming stores, which
+// keep them from evicting the tiles and cut the misses by 14 %.
 template <int N, int TW = Cfg<N>::TW>
 __global__ __launch_bounds__((TileMap<N, TW>::NT)) __attribute__((amdgpu_waves_per_eu(4))) void k_ix_filter_k(
   const float2* __restrict__ U, const float2* __restrict__ V, float2* __restrict__ T,
