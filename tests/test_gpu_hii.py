@@ -115,13 +115,47 @@ def test_spin_temperature_field(ctx):
 
 
 def test_config2_like_128_sharpk(ctx):
-    """Sharp-k with UVB on a 128^3, 200 Mpc/h box (the 256^3 case runs in the slow set)."""
+    """Sharp-k with UVB on a 128^3, 200 Mpc/h box (the 256^3 case follows)."""
     N = 128
     d, s, w = synth.snapshot_inputs(N, 0)
     p = P.hii_params(N, 200.0, 9.0, filter_type=1, efficiency=20.0, uvb_flag=1)
     g, o = run_pair(ctx, p, d, s, w)
     check_parity(g, o, N, "C2-like 128")
 
+
+def test_config2_256_sharpk(ctx):
+    """The 256^3, 200 Mpc/h, z = 9 configuration itself: sharp-k, UVB on, J_21
+    out, 44 radii.  The parity contract as in check_parity; on the partially
+    neutral cells (n_f - eff f_coll from the float round trip of the
+    unfiltered fields) the HIP path must be as close to the FP64 oracle as the
+    float oracle is.  The two oracle runs take about a minute on the host."""
+    N = 256
+    print(f"config 2 at {N}^3: GPU, float and FP64 oracle", flush=True)
+    d, s, w = synth.snapshot_inputs(N, 0)
+    p = P.hii_params(N, 200.0, 9.0, filter_type=1, efficiency=20.0, uvb_flag=1)
+    g, o = run_pair(ctx, p, d, s, w)
+    o64 = O.find_hii_bubbles(p, d, s, w, precision=64)
+    assert g["n_radii"] == o["n_radii"] == 44
+    dxh, dmass, frac, worst, flips = compare(g, o, N, "C2 256")
+    assert dxh <= 1e-5 and dmass <= 1e-5 and frac <= 1e-4 and worst <= 1e-4
+    keep = ~flips
+    ion = (o["xH"] < 1e-5) & keep
+    assert np.array_equal(g["xH"][ion], o["xH"][ion])
+    assert np.array_equal(g["r_bubble"][ion], o["r_bubble"][ion])
+    zi = keep & (o["z_at_ionization"] >= 0)
+    assert np.array_equal(g["z_at_ionization"][zi], o["z_at_ionization"][zi])
+    neutral = (g["xH"] >= 1e-5) & (o["xH"] >= 1e-5) & (o64["xH"] >= 1e-5)
+    e_g = float(np.abs(g["xH"][neutral].astype(np.float64) - o64["xH"][neutral]).max())
+    e_32 = float(np.abs(o["xH"][neutral].astype(np.float64) - o64["xH"][neutral]).max())
+    print(f"C2 256 partial xH vs FP64: gpu {e_g:.2e}, float oracle {e_32:.2e}", flush=True)
+    assert e_g <= 2 * e_32 + 1e-6
+    # J_21 where both sides ionised the cell, as in test_uvb_and_carried_state
+    dJ = np.abs(g["J_21"][ion].astype(np.float64) - o["J_21"][ion])
+    scale = float(np.sqrt(np.mean(o["J_21"][ion].astype(np.float64) ** 2)))
+    print(f"C2 256 J_21 max |d|/rms {dJ.max() / scale:.2e}", flush=True)
+    assert (dJ <= 1e-4 * np.abs(o["J_21"][ion]) + 1e-6 * scale).mean() > 0.999
+    assert abs(g["volume_weighted_global_J_21"] - o["volume_weighted_global_J_21"]) <= 1e-4 * abs(
+        o["volume_weighted_global_J_21"]) + 1e-30
 
 def test_device_pointers_match_host(ctx):
     import torch
