@@ -38,6 +38,7 @@ SPECS = [
     (64, 100.0, 9.0, 0, 11.0, 1, 4),
     (64, 100.0, 8.5, 0, 38.0, 1, 5),
     (128, 100.0, 8.0, 1, 25.0, 0, 6),
+    (192, 100.0, 8.0, 0, 17.0, 1, 7),
 ]
 
 KEYS = ("xH", "r_bubble", "J_21", "z_at_ionization", "J_21_at_ionization")
@@ -63,8 +64,9 @@ def test_untuned_batch_equals_single_box_calls(ctx, graph):
     batch = reion.HiiBatch(ctx, boxes, graph=graph, tune=False)
     info = batch.describe()
     print(info, flush=True)
-    # the three 64^3 boxes differ only in z and efficiency: one cohort
-    assert len(info["cohorts"]) == 3
+    # the three 64^3 boxes differ only in z and efficiency: one cohort, and
+    # one each for the 32^3, 128^3 and 192^3 boxes
+    assert len(info["cohorts"]) == 4
     assert info["graph"] == graph
     refs = [single(ctx, b) for b in batch.boxes]
     for rep in range(3):
