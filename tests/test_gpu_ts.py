@@ -23,7 +23,7 @@ def rel(a, b):
     return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
 
 
-@pytest.mark.parametrize("N,L", [(32, 25.0), (64, 50.0)])
+@pytest.mark.parametrize("N,L", [(32, 25.0), (64, 50.0), (128, 100.0)])
 def test_ts_snapshot_sequence(ctx, N, L):
     run = SyntheticRun(N, L, seed=3)
     nh = 8
