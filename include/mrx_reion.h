@@ -254,6 +254,21 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
                         const float* TS_box, float* delta_T, double* ave_Tb);
 
 /* ---------------------------------------------------------------------------
+ * No counterpart in the reference.  mrx_brightness_temp for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it allocates nothing, host or
+ * device.  Its partial sums live in a fixed device array of 2056 doubles
+ * (16 KiB + 64 B) that each device holds until the process ends; that size
+ * does not depend on the grid.  Device pointers only (p->pointers_on_device
+ * = 1, else MRX_EINVAL); grid_dim even, 2 <= grid_dim <= 2048 (above:
+ * MRX_EUNSUPPORTED).  Results equal mrx_brightness_temp's bit for bit.  All
+ * calls on one device share that array: calls that overlap on one device
+ * (other contexts or streams) race, so serialise them.
+ */
+int mrx_brightness_temp_noalloc(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, const float* deltax,
+                                const float* TS_box, float* delta_T, double* ave_Tb);
+
+/* ---------------------------------------------------------------------------
  * Kernel timing, the device-side counterpart of the reference's timer_info
  * (src/core/utils.c:205-220) around find_HII_bubbles / ComputeTs
  * (find_HII_bubbles.c:498-531).  While enabled, every kernel launch is

@@ -20,7 +20,7 @@ EXPORTS = (
     "mrx_brightness_temp", "mrx_abi_sizeof", "mrx_profile", "mrx_profile_read", "mrx_compute_ts",
     "mrx_hii_batch_create", "mrx_hii_batch_run", "mrx_hii_batch_describe", "mrx_hii_batch_free",
     "mrx_power_spectrum", "mrx_power_spectrum_adjoint", "mrx_snapshot_adjoint", "mrx_construct_baryon_grids",
-    "mrx_brightness_temp_vel", "mrx_find_hii_bubbles_rec",
+    "mrx_brightness_temp_vel", "mrx_find_hii_bubbles_rec", "mrx_brightness_temp_noalloc",
 )
 
 # every symbol include/mrx_io.h declares (libmrx_io.so)
@@ -59,6 +59,7 @@ def load():
     L.mrx_fft_c2r_3d.argtypes = [ctxp, i, vp, vp, i]
     L.mrx_filtered_field.argtypes = [ctxp, i, d, i, vp, d, i, i, vp, i]
     L.mrx_brightness_temp.argtypes = [ctxp, C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
+    L.mrx_brightness_temp_noalloc.argtypes = [ctxp, C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
     L.mrx_compute_ts.argtypes = [ctxp, C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals)]
     L.mrx_profile.argtypes = [ctxp, i]
     L.mrx_profile_read.argtypes = [ctxp, i, C.c_char_p, i, vp, vp]

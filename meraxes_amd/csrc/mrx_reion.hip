@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,37 @@ constexpr double kProtonMass = 1.6726e-24;
 constexpr double kTcmb = 2.728;
 constexpr double kLFactor = 0.620350491;
 constexpr int kMaxSteps = 255;  // the per-cell step index is one byte
+
+// Workgroups of k_brightness_temp (grid-stride): one partial sum each, for
+// both brightness-temperature entry points, so their sums are formed alike.
+constexpr int kTbBlocks = 2048;
+
+// Device scratch of mrx_brightness_temp_noalloc: the kTbBlocks partial sums of
+// k_brightness_temp, then the reduced total.  Its size does not depend on the
+// grid.  Every device holds its own copy until the process ends.  Host code
+// never uses the array's name (on the host it is only a stand-in symbol): it
+// uses the device address resolved once per device by resolve_scratch().
+constexpr int kNoallocScratch = kTbBlocks + 8;  // doubles (16 KiB + 64 B)
+constexpr int kNoallocMaxDim = 2048;            // largest grid_dim the no-allocation entry point accepts
+static __device__ __align__(256) double scratch[kNoallocScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves this device's copy of `scratch`, with `device` current; called from
+// mrx_init.  Returns the address or nullptr (then only the no-allocation entry
+// point is unavailable, and it says so).
+double* resolve_scratch(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    double* ptr = nullptr;
+    if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
 }  // namespace
 
 // Radii per inverse x pass of the bubble ladder (1..kLadderRadiiMax); the
@@ -108,6 +140,8 @@ struct mrx_ctx {
   // buffers of the other translation units (ctx_buffer)
   void* ext[mrx::kCtxSlots] = {};
   size_t ext_cap[mrx::kCtxSlots] = {};
+  // this device's copy of the file-scope `scratch` array (mrx_brightness_temp_noalloc)
+  double* noalloc_scratch = nullptr;
 };
 
 namespace {
@@ -174,20 +208,20 @@ int check_launch(mrx_ctx* ctx, const char* what)
 // Sums the nb x nv row-major partials of each of nbox boxes (boxes
 // contiguous) into out[box * nv + v]: in one workgroup per box for short
 // arrays, otherwise in two levels (k_reduce_rows, then one workgroup per box)
-// through `scratch`, kRedBlocks * kRedMaxV * nbox doubles.  The order of the
+// through `rows`, kRedBlocks * kRedMaxV * nbox doubles.  The order of the
 // additions depends only on (nb, nv), never on timing.
-int reduce_partials(mrx_ctx* ctx, hipStream_t st, const double* partials, int nb, int nv, int nbox, double* scratch,
+int reduce_partials(mrx_ctx* ctx, hipStream_t st, const double* partials, int nb, int nv, int nbox, double* rows,
                     double* out)
 {
-  if (nv > kRedMaxV || nb < 32 * kNT || !scratch) {
+  if (nv > kRedMaxV || nb < 32 * kNT || !rows) {
     k_reduce_partials<<<nbox, kNT, 0, st>>>(partials, nb, nv, out);
     return check_launch(ctx, "k_reduce_partials");
   }
   const int G = std::min(kRedBlocks, (nb + kNT - 1) / kNT);
   const int chunk = (nb + G - 1) / G;
-  k_reduce_rows<<<dim3(G, nbox), kNT, 0, st>>>(partials, nb, nv, chunk, scratch);
+  k_reduce_rows<<<dim3(G, nbox), kNT, 0, st>>>(partials, nb, nv, chunk, rows);
   MRX_TRY(check_launch(ctx, "k_reduce_rows"));
-  k_reduce_partials<<<nbox, kNT, 0, st>>>(scratch, G, nv, out);
+  k_reduce_partials<<<nbox, kNT, 0, st>>>(rows, G, nv, out);
   return check_launch(ctx, "k_reduce_partials");
 }
 
@@ -1579,6 +1613,7 @@ int mrx_init(int device, mrx_ctx** out)
     return MRX_EDEVICE;
   }
   ctx->stream = ctx->own_stream;
+  ctx->noalloc_scratch = resolve_scratch(device);
   if (const char* e = getenv("MRX_LADDER_RADII")) {
     const int k = atoi(e);
     if (k >= 1 && k <= kLadderRadiiMax) ctx->ladder_radii = k;
@@ -1912,7 +1947,7 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
     dTS = p->include_spin_temp ? s + np + nr : nullptr;
     ddT = s + np + 2 * nr;
   }
-  const int nb = 2048;
+  const int nb = kTbBlocks;
   MRX_TRY(ensure_partials(ctx, nb + 8));
   if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
   {
@@ -1926,6 +1961,38 @@ int mrx_brightness_temp(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, c
   HIP_TRY(hipMemcpyAsync(&sum, ctx->sums, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   if (!p->pointers_on_device)
     HIP_TRY(hipMemcpyAsync(delta_T, ddT, sizeof(float) * nr, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ave_Tb) *ave_Tb = sum / pow((double)N, 3);
+  return MRX_OK;
+}
+
+int mrx_brightness_temp_noalloc(mrx_ctx* ctx, const mrx_tb_params* p, const float* xH, const float* deltax,
+                                const float* TS_box, float* delta_T, double* ave_Tb)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!p || !xH || !deltax || !delta_T) return set_err(ctx, MRX_EINVAL, "bad brightness_temp_noalloc arguments");
+  if (!p->pointers_on_device)
+    return set_err(ctx, MRX_EINVAL, "brightness_temp_noalloc takes device pointers only (host grids need staging)");
+  if (p->grid_dim < 2 || (p->grid_dim & 1))
+    return set_err(ctx, MRX_EINVAL, "brightness_temp_noalloc: grid_dim %d must be even and >= 2", p->grid_dim);
+  if (p->grid_dim > kNoallocMaxDim)
+    return set_err(ctx, MRX_EUNSUPPORTED, "brightness_temp_noalloc: grid_dim %d is above its maximum %d",
+                   p->grid_dim, kNoallocMaxDim);
+  if (p->include_spin_temp && !TS_box) return set_err(ctx, MRX_EINVAL, "include_spin_temp needs TS_box");
+  double* sp = ctx->noalloc_scratch;
+  if (!sp) return set_err(ctx, MRX_EDEVICE, "brightness_temp_noalloc: the device scratch address was not resolved");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int N = p->grid_dim;
+  const TbArgs a = tb_args(p);
+  // every partial is written by its workgroup before the reduction reads it, and
+  // the total by the reduction: nothing in `scratch` needs clearing first
+  k_brightness_temp<<<kTbBlocks, kNT, 0, ctx->stream>>>(xH, deltax, p->include_spin_temp ? TS_box : nullptr,
+                                                        delta_T, sp, (long)nreal(N), a);
+  MRX_TRY(check_launch(ctx, "k_brightness_temp"));
+  k_reduce_partials<<<1, kNT, 0, ctx->stream>>>(sp, kTbBlocks, 1, sp + kTbBlocks);
+  MRX_TRY(check_launch(ctx, "k_reduce_partials"));
+  double sum = 0;
+  HIP_TRY(hipMemcpyAsync(&sum, sp + kTbBlocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (ave_Tb) *ave_Tb = sum / pow((double)N, 3);
   return MRX_OK;
