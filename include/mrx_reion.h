@@ -245,6 +245,52 @@ int mrx_compute_ts(mrx_ctx* ctx, const mrx_ts_params* p, const float* deltax, co
                    mrx_ts_globals* out);
 
 /* ---------------------------------------------------------------------------
+ * The host tables of one spin-temperature snapshot: what _ComputeTs forms on
+ * the host before its per-cell loop (ComputeTs.c:276-377 shell radii,
+ * light-cone edges and SFR-history operations; :517-716 frequency integrals
+ * at the 14 x_e knots from the lower limit nu(tau_X = 1)
+ * (XRayHeatingFunctions.c:228-375, :685-803), Ly-n sums with the kink fix,
+ * growth factor, X-ray prefactor; evolveInt's per-shell factors
+ * XRayHeatingFunctions.c:1289-1311).  mrx_compute_ts uses exactly these.  Host
+ * only, no device work: for callers that log or check the tables.
+ * collapse_fraction and x_e_ave are the two grid means the tables depend on
+ * (ComputeTs.c:413-475); like the reference, p->stored_fcoll[p->snapshot]
+ * receives collapse_fraction (inhomogeneous snapshots only).  Operations are
+ * listed in plan order: op_kind 0 zeroes the first N*N*(N/2+1) floats of the
+ * working grid, 1 scales them by op_weight, 2 loads history slot op_slot
+ * times (float)op_weight, 3 adds it.  More than MRX_TS_TABLES_MAX_OPS
+ * operations in all, or a shell needing more than 256, is MRX_EUNSUPPORTED.
+ */
+#define MRX_TS_TABLES_MAX_STEPS 64
+#define MRX_TS_TABLES_MAX_OPS 1024
+typedef struct mrx_ts_tables {
+  int nsteps, homogeneous, no_light, n_ops;
+  double zp, dzp, dt_dzp, const_zp_prefactor, growth_factor_zp, dgrowth_factor_dzp;
+  double R[MRX_TS_TABLES_MAX_STEPS];        /* shell radii [Mpc]                               */
+  double zpp_edge[MRX_TS_TABLES_MAX_STEPS]; /* light-cone redshift edges                       */
+  double zpp[MRX_TS_TABLES_MAX_STEPS], dzpp[MRX_TS_TABLES_MAX_STEPS], dt_dzpp[MRX_TS_TABLES_MAX_STEPS];
+  double zpp_pow[MRX_TS_TABLES_MAX_STEPS];  /* (1 + z'')^-SpecIndexXrayGal                     */
+  double sum_lyn[MRX_TS_TABLES_MAX_STEPS];
+  double nu_lower[MRX_TS_TABLES_MAX_STEPS]; /* max(nu(tau_X = 1), NuXrayGalThreshold) [Hz]     */
+  double freq_heat[14][MRX_TS_TABLES_MAX_STEPS], freq_ion[14][MRX_TS_TABLES_MAX_STEPS];
+  double freq_lya[14][MRX_TS_TABLES_MAX_STEPS];
+  int op_shell[MRX_TS_TABLES_MAX_OPS], op_kind[MRX_TS_TABLES_MAX_OPS], op_slot[MRX_TS_TABLES_MAX_OPS];
+  double op_weight[MRX_TS_TABLES_MAX_OPS];
+} mrx_ts_tables;
+
+int mrx_ts_snapshot_tables(const mrx_ts_params* p, double collapse_fraction, double x_e_ave, mrx_ts_tables* out);
+
+/* The smoothed SFR shells of the context's last mrx_compute_ts call on the
+ * inhomogeneous branch: per shell max(filtered sfr, 0) in float
+ * (ComputeTs.c:379-503), before the unit factor the reference folds into its
+ * double SMOOTHED_SFR_GAL (:419-421, :487-489).  Shell-major, unpadded:
+ * out receives n_shells * grid_dim^3 floats (device memory when
+ * pointers_on_device).  MRX_EINVAL when the last call left no shells (the
+ * homogeneous branch) or the sizes differ.  The reference keeps these only
+ * inside run_globals; this is for callers that inspect them. */
+int mrx_ts_last_shells(mrx_ctx* ctx, int grid_dim, int n_shells, float* out, int pointers_on_device);
+
+/* ---------------------------------------------------------------------------
  * Brightness temperature.  Drop-in for ComputeBrightnessTemperatureBox(snapshot)
  * (BrightnessTemperature.c:24-124, :512-545; declared BrightnessTemperature.h:9)
  * for Flag_IncludePecVelsFor21cm = 0.  TS_box is read only with
@@ -458,7 +504,7 @@ int mrx_construct_baryon_grids(mrx_ctx* ctx, const mrx_grid_params* p, long n_ga
 
 /* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params,
  * 3 mrx_ts_params, 4 mrx_ts_globals, 5 mrx_hii_batch_box, 6 mrx_grid_params,
- * 7 mrx_grid_stats, 8 mrx_tb_vel_params), so a binding can check that its
+ * 7 mrx_grid_stats, 8 mrx_tb_vel_params, 9 mrx_ts_tables), so a binding can check that its
  * mirror of the layout matches this build. */
 size_t mrx_abi_sizeof(int which);
 

@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 from .abi import (ERRORS, GridParams, GridStats, HiiBatchBox, HiiGlobals, HiiParams, OutputGrids, SaveParams,
-                  TbParams, TbVelParams, TsGlobals, TsParams)
+                  TbParams, TbVelParams, TsGlobals, TsParams, TsTables)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmrx_hip.so")
@@ -20,7 +20,8 @@ EXPORTS = (
     "mrx_brightness_temp", "mrx_abi_sizeof", "mrx_profile", "mrx_profile_read", "mrx_compute_ts",
     "mrx_hii_batch_create", "mrx_hii_batch_run", "mrx_hii_batch_describe", "mrx_hii_batch_free",
     "mrx_power_spectrum", "mrx_power_spectrum_adjoint", "mrx_snapshot_adjoint", "mrx_construct_baryon_grids",
-    "mrx_brightness_temp_vel", "mrx_find_hii_bubbles_rec", "mrx_brightness_temp_noalloc",
+    "mrx_brightness_temp_vel", "mrx_find_hii_bubbles_rec", "mrx_brightness_temp_noalloc", "mrx_ts_snapshot_tables",
+    "mrx_ts_last_shells",
 )
 
 # every symbol include/mrx_io.h declares (libmrx_io.so)
@@ -61,6 +62,8 @@ def load():
     L.mrx_brightness_temp.argtypes = [ctxp, C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
     L.mrx_brightness_temp_noalloc.argtypes = [ctxp, C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
     L.mrx_compute_ts.argtypes = [ctxp, C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals)]
+    L.mrx_ts_snapshot_tables.argtypes = [C.POINTER(TsParams), d, d, C.POINTER(TsTables)]
+    L.mrx_ts_last_shells.argtypes = [ctxp, i, i, vp, i]
     L.mrx_profile.argtypes = [ctxp, i]
     L.mrx_profile_read.argtypes = [ctxp, i, C.c_char_p, i, vp, vp]
     L.mrx_hii_batch_create.argtypes = [ctxp, C.POINTER(HiiBatchBox), i, i, C.POINTER(vp)]

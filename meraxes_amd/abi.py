@@ -224,6 +224,26 @@ class TbVelParams(C.Structure):
     ]
 
 
+_TSM, _TSO = 64, 1024  # MRX_TS_TABLES_MAX_STEPS, MRX_TS_TABLES_MAX_OPS
+
+
+class TsTables(C.Structure):
+    """mrx_ts_tables: the host tables of one spin-temperature snapshot."""
+
+    _fields_ = [
+        ("nsteps", C.c_int), ("homogeneous", C.c_int), ("no_light", C.c_int), ("n_ops", C.c_int),
+        ("zp", C.c_double), ("dzp", C.c_double), ("dt_dzp", C.c_double), ("const_zp_prefactor", C.c_double),
+        ("growth_factor_zp", C.c_double), ("dgrowth_factor_dzp", C.c_double),
+        ("R", C.c_double * _TSM), ("zpp_edge", C.c_double * _TSM), ("zpp", C.c_double * _TSM),
+        ("dzpp", C.c_double * _TSM), ("dt_dzpp", C.c_double * _TSM), ("zpp_pow", C.c_double * _TSM),
+        ("sum_lyn", C.c_double * _TSM), ("nu_lower", C.c_double * _TSM),
+        ("freq_heat", (C.c_double * _TSM) * 14), ("freq_ion", (C.c_double * _TSM) * 14),
+        ("freq_lya", (C.c_double * _TSM) * 14),
+        ("op_shell", C.c_int * _TSO), ("op_kind", C.c_int * _TSO), ("op_slot", C.c_int * _TSO),
+        ("op_weight", C.c_double * _TSO),
+    ]
+
+
 class SaveParams(C.Structure):
     """mrx_save_params (include/mrx_io.h): flags and units of the grid files."""
 

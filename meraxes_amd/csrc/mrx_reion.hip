@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -118,6 +119,7 @@ struct mrx_ctx {
   size_t ts_work_cap = 0;
   float* ts_shells = nullptr;
   size_t ts_shells_cap = 0;
+  int ts_shells_N = 0, ts_shells_n = 0;  // grid and shell count ts_shells holds (0: none)
   float* wtab = nullptr;  // window tables of the current radii, (N/2+1)^3 floats each
   size_t wtab_cap = 0;
   // the bubble ladder's inverse x pass covers ladder_radii radii per launch
@@ -1066,11 +1068,13 @@ struct TsRun {
         }
         nred = nred_h;
         MRX_TRY(check_launch(ctx, "k_ts_homogeneous"));
+        ctx->ts_shells_N = ctx->ts_shells_n = 0;  // no shells on the homogeneous branch
         MRX_TRY(fcoll(kLFactor * box_size / (float)N, 0));
         x_e_ave = 0.0;
       } else {
         MRX_TRY(ensure_buffers(ctx, N, 1));
         MRX_TRY(ensure_buf(ctx, ctx->ts_work, ctx->ts_work_cap, np));
+        ctx->ts_shells_N = ctx->ts_shells_n = 0;
         MRX_TRY(ensure_buf(ctx, ctx->ts_shells, ctx->ts_shells_cap, (size_t)s->nsteps * ncell));
         // f_coll and <x_e> of the first shell (ComputeTs.c:413-475) read the
         // input grids only: taken first, so the host builds the tables while
@@ -1081,14 +1085,17 @@ struct TsRun {
         // until the first shell with operations writes ts_work from it
         const float* wgrid = sfr;
         for (int ct = 0; ct < s->nsteps; ++ct) {
-          if (s->n_ops[ct] > 0) {
+          // a shell's operations in plan order, MRX_TS_MAX_OPS per launch (each
+          // launch after the first reads the working grid the previous wrote)
+          for (int k0 = 0; k0 < s->n_ops[ct]; k0 += MRX_TS_MAX_OPS) {
             SfrOps o;
-            o.n = s->n_ops[ct];
+            o.n = std::min(MRX_TS_MAX_OPS, s->n_ops[ct] - k0);
             for (int k = 0; k < o.n; ++k) {
-              o.kind[k] = s->ops[ct][k].kind;
-              o.slot[k] = s->ops[ct][k].slot;
-              o.wd[k] = s->ops[ct][k].weight;
-              o.wf[k] = (float)s->ops[ct][k].weight;
+              const mrx_sfr_op& op = s->ops[ct][k0 + k];
+              o.kind[k] = op.kind;
+              o.slot[k] = op.slot;
+              o.wd[k] = op.weight;
+              o.wf[k] = (float)op.weight;
             }
             PROF("sfr_op");
             k_sfr_ops<<<(unsigned)((np / 2 + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->ts_work, wgrid, hist, (long)np,
@@ -1113,6 +1120,8 @@ struct TsRun {
           MRX_TRY(P::inverse_xy(ctx, ctx->U, ctx->V, ctx->T, 1, w, kzlive));
           MRX_TRY(P::c2r_z(ctx, ctx->T, ctx->ts_shells + (size_t)ct * ncell, 1, 3, N, kzlive));
         }
+        ctx->ts_shells_N = N;
+        ctx->ts_shells_n = s->nsteps;
         if (p->fixed_globals) {
           collapse_fraction = p->fixed_collapse_fraction;
           x_e_ave = p->fixed_x_e_ave;
@@ -1879,7 +1888,11 @@ int mrx_compute_ts(mrx_ctx* ctx, const mrx_ts_params* p, const float* deltax, co
   }
   if (!ctx->tss) ctx->tss = new mrx_ts_snap();
   if (!ctx->ts_host) ctx->ts_host = new TsDev();
-  if (mrx_ts_plan(ctx->tss, ctx->tsd, p) != MRX_OK) return set_err(ctx, MRX_EINVAL, "bad spin-temperature plan");
+  if (const int prc = mrx_ts_plan(ctx->tss, ctx->tsd, p); prc != MRX_OK)
+    return prc == MRX_EUNSUPPORTED
+             ? set_err(ctx, prc, "a spin-temperature shell needs more than %d SFR-history operations",
+                       MRX_TS_MAX_SHELL_OPS)
+             : set_err(ctx, MRX_EINVAL, "bad spin-temperature plan");
   const mrx_ts_snap* s = ctx->tss;
   if (!s->homogeneous && s->max_slot >= 0 && (!sfr_histories || s->max_slot >= p->n_history))
     return set_err(ctx, MRX_EINVAL, "the shells need SFR history slot %d, %d held", s->max_slot, p->n_history);
@@ -2146,7 +2159,11 @@ int mrx_snapshot_adjoint(mrx_ctx* ctx, const mrx_ts_params* ts, const mrx_hii_pa
     }
     if (!ctx->tss) ctx->tss = new mrx_ts_snap();
     if (!ctx->ts_host) ctx->ts_host = new TsDev();
-    if (mrx_ts_plan(ctx->tss, ctx->tsd, ts) != MRX_OK) return set_err(ctx, MRX_EINVAL, "bad spin-temperature plan");
+    if (const int prc = mrx_ts_plan(ctx->tss, ctx->tsd, ts); prc != MRX_OK)
+      return prc == MRX_EUNSUPPORTED
+               ? set_err(ctx, prc, "a spin-temperature shell needs more than %d SFR-history operations",
+                         MRX_TS_MAX_SHELL_OPS)
+               : set_err(ctx, MRX_EINVAL, "bad spin-temperature plan");
     const mrx_ts_snap* s = ctx->tss;
     if (!s->homogeneous && s->max_slot >= 0 && (!sfr_histories || s->max_slot >= ts->n_history))
       return set_err(ctx, MRX_EINVAL, "the shells need SFR history slot %d, %d held", s->max_slot, ts->n_history);
@@ -2295,6 +2312,73 @@ int mrx_hii_batch_free(mrx_hii_batch* b)
 }
 
 // Size of each ABI struct, so bindings can check their mirrors.
+int mrx_ts_last_shells(mrx_ctx* ctx, int grid_dim, int n_shells, float* out, int pointers_on_device)
+{
+  if (!ctx) return MRX_EINVAL;
+  if (!out) return set_err(ctx, MRX_EINVAL, "ts_last_shells: null output");
+  if (ctx->ts_shells_n == 0 || ctx->ts_shells_N != grid_dim || ctx->ts_shells_n != n_shells)
+    return set_err(ctx, MRX_EINVAL, "ts_last_shells: the context holds %d shells of %d^3 (asked %d of %d^3)",
+                   ctx->ts_shells_n, ctx->ts_shells_N, n_shells, grid_dim);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t bytes = sizeof(float) * (size_t)n_shells * nreal(grid_dim);
+  HIP_TRY(hipMemcpyAsync(out, ctx->ts_shells, bytes, pointers_on_device ? hipMemcpyDeviceToDevice
+                                                                         : hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MRX_OK;
+}
+
+int mrx_ts_snapshot_tables(const mrx_ts_params* p, double collapse_fraction, double x_e_ave, mrx_ts_tables* out)
+{
+  if (!p || !out || !p->tables_dir) return MRX_EINVAL;
+  if (p->num_filter_steps < 1 || p->num_filter_steps > MRX_TS_TABLES_MAX_STEPS) return MRX_EUNSUPPORTED;
+  std::unique_ptr<mrx_ts_data> d(new mrx_ts_data());
+  std::unique_ptr<mrx_ts_snap> s(new mrx_ts_snap());
+  if (mrx_ts_data_init(d.get(), p) != MRX_OK) return MRX_EINVAL;
+  const int rc = mrx_ts_plan(s.get(), d.get(), p);
+  if (rc != MRX_OK) return rc;
+  int nop = 0;
+  for (int c = 0; c < s->nsteps; ++c) nop += s->n_ops[c];
+  if (nop > MRX_TS_TABLES_MAX_OPS) return MRX_EUNSUPPORTED;
+  memset(out, 0, sizeof *out);
+  out->nsteps = s->nsteps;
+  out->homogeneous = s->homogeneous;
+  out->zp = s->zp;
+  out->dzp = s->dzp;
+  out->dt_dzp = s->dt_dzp;
+  if (s->homogeneous) return MRX_OK;
+  s->collapse_fraction = collapse_fraction;
+  s->x_e_ave = x_e_ave;
+  MRX_TRY(mrx_ts_build_tables(s.get(), d.get(), p));
+  out->no_light = s->no_light;
+  out->dt_dzp = s->dt_dzp;
+  out->const_zp_prefactor = s->const_zp_prefactor;
+  out->growth_factor_zp = s->growth_factor_zp;
+  out->dgrowth_factor_dzp = s->dgrowth_factor_dzp;
+  for (int c = 0; c < s->nsteps; ++c) {
+    out->R[c] = s->R[c];
+    out->zpp_edge[c] = s->zpp_edge[c];
+    out->zpp[c] = s->zpp[c];
+    out->dzpp[c] = s->dzpp[c];
+    out->dt_dzpp[c] = s->dt_dzpp[c];
+    out->zpp_pow[c] = s->zpp_pow[c];
+    out->sum_lyn[c] = s->sum_lyn[c];
+    out->nu_lower[c] = s->nu_lower[c];
+    for (int x = 0; x < MRX_X_INT_NXHII; ++x) {
+      out->freq_heat[x][c] = s->freq_heat[x][c];
+      out->freq_ion[x][c] = s->freq_ion[x][c];
+      out->freq_lya[x][c] = s->freq_lya[x][c];
+    }
+    for (int k = 0; k < s->n_ops[c]; ++k) {
+      const int i = out->n_ops++;
+      out->op_shell[i] = c;
+      out->op_kind[i] = s->ops[c][k].kind;
+      out->op_slot[i] = s->ops[c][k].slot;
+      out->op_weight[i] = s->ops[c][k].weight;
+    }
+  }
+  return MRX_OK;
+}
+
 size_t mrx_abi_sizeof(int which)
 {
   switch (which) {
@@ -2307,6 +2391,7 @@ size_t mrx_abi_sizeof(int which)
     case 6: return sizeof(mrx_grid_params);
     case 7: return sizeof(mrx_grid_stats);
     case 8: return sizeof(mrx_tb_vel_params);
+    case 9: return sizeof(mrx_ts_tables);
     default: return 0;
   }
 }

@@ -671,8 +671,11 @@ double mrx_ts_brent(double (*f)(double, void*), void* par, double lo, double hi,
     else
       b += (m > 0 ? +tol : -tol);
     fb = f(b, par);
-    /* bracket [min(b,c), max(b,c)] within relerr of its smaller-magnitude end */
-    const double xl = fmin(b, c), xu = fmax(b, c);
+    /* The bracket this iteration reports: the new point b and the end of
+     * opposite sign - c, or a (the previous b) when f(b) has c's sign.  The
+     * search stops when it is within relerr of its smaller-magnitude end. */
+    const double cb = ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) ? a : c;
+    const double xl = fmin(b, cb), xu = fmax(b, cb);
     const double min_abs = (xl > 0 && xu > 0) || (xl < 0 && xu < 0) ? fmin(fabs(xl), fabs(xu)) : 0;
     if (fabs(xu - xl) < relerr * min_abs) return b;
   }
@@ -894,11 +897,13 @@ double mrx_ts_integrate_over_nu(const mrx_ts_data* d, const mrx_ts_params* p, do
 
 static void push_op(mrx_ts_snap* s, int ct, int kind, int slot, double w)
 {
-  if (s->n_ops[ct] < MRX_TS_MAX_OPS) {
+  if (s->n_ops[ct] < MRX_TS_MAX_SHELL_OPS) {
     mrx_sfr_op* o = &s->ops[ct][s->n_ops[ct]++];
     o->kind = kind;
     o->slot = slot;
     o->weight = w;
+  } else {
+    s->ops_overflow = 1;
   }
   if ((kind == MRX_SFR_LOAD || kind == MRX_SFR_ADD) && slot > s->max_slot) s->max_slot = slot;
 }
@@ -991,6 +996,7 @@ int mrx_ts_plan(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p)
     R *= R_factor;
   }
   s->final_weight = weight;
+  if (s->ops_overflow) return MRX_EUNSUPPORTED;
   /* per-shell factors of evolveInt (XRayHeatingFunctions.c:1291-1303) */
   for (int ct = 0; ct < nsteps; ct++) {
     if (ct == 0) {
@@ -1029,6 +1035,7 @@ static void tab_item(tab_job* j, int ct, nu_cache* cache)
   const double zpp = (s->zpp_edge[ct] + prev_zpp) * 0.5;
   const double lower =
     fmax(mrx_ts_nu_tau_one(d, p, zp, zpp, s->x_e_ave, s->x_e_ave), p->nu_xray_gal_threshold * K_NU_over_EV);
+  s->nu_lower[ct] = lower;
   double sum = 0;
   for (int n_ct = MRX_NSPEC_MAX; n_ct >= 2; n_ct--) {
     if (zpp > zmax((float)zp, n_ct)) continue;

@@ -19,7 +19,8 @@ extern "C" {
 #endif
 
 #define MRX_TS_MAX_STEPS 64
-#define MRX_TS_MAX_OPS 24
+#define MRX_TS_MAX_OPS 24        /* SFR-history operations per k_sfr_ops launch (a by-value kernel argument) */
+#define MRX_TS_MAX_SHELL_OPS 256 /* operations one shell may need; a plan needing more is MRX_EUNSUPPORTED */
 #define MRX_X_INT_NXHII 14
 #define MRX_X_INT_NENERGY 258
 #define MRX_NSPEC_MAX 23
@@ -79,7 +80,8 @@ typedef struct mrx_ts_snap {
   double R[MRX_TS_MAX_STEPS];        /* Mpc */
   double zpp_edge[MRX_TS_MAX_STEPS];
   int n_ops[MRX_TS_MAX_STEPS];
-  mrx_sfr_op ops[MRX_TS_MAX_STEPS][MRX_TS_MAX_OPS];
+  mrx_sfr_op ops[MRX_TS_MAX_STEPS][MRX_TS_MAX_SHELL_OPS];
+  int ops_overflow; /* a shell needed more than MRX_TS_MAX_SHELL_OPS operations */
   int max_slot; /* largest history slot read */
   double final_weight; /* `weight` after the shell loop, reused by the Ly-n kink fix */
   /* after the shell pass */
@@ -89,6 +91,7 @@ typedef struct mrx_ts_snap {
   double freq_ion[MRX_X_INT_NXHII][MRX_TS_MAX_STEPS];
   double freq_lya[MRX_X_INT_NXHII][MRX_TS_MAX_STEPS];
   double sum_lyn[MRX_TS_MAX_STEPS];
+  double nu_lower[MRX_TS_MAX_STEPS]; /* lower frequency limit: max(nu(tau_X = 1), NuXrayGalThreshold) */
   double const_zp_prefactor, growth_factor_zp, dgrowth_factor_dzp;
   /* per-shell factors of evolveInt's z'' sum (XRayHeatingFunctions.c:1289-1311) */
   double zpp[MRX_TS_MAX_STEPS], dzpp[MRX_TS_MAX_STEPS], dt_dzpp[MRX_TS_MAX_STEPS];
@@ -98,7 +101,8 @@ typedef struct mrx_ts_snap {
 /* Loads the tables from `dir` and sets the cosmology; 0 or a negative code. */
 int mrx_ts_data_init(mrx_ts_data* d, const mrx_ts_params* p);
 
-/* Shell radii, light-cone edges and SFR-history operations (needs no grids). */
+/* Shell radii, light-cone edges and SFR-history operations (needs no grids);
+ * MRX_EUNSUPPORTED when a shell needs more than MRX_TS_MAX_SHELL_OPS operations. */
 int mrx_ts_plan(mrx_ts_snap* s, const mrx_ts_data* d, const mrx_ts_params* p);
 
 /* Frequency-integral tables, sum_lyn and prefactors, once collapse_fraction and

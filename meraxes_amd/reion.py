@@ -32,7 +32,7 @@ import numpy as np
 
 from . import _lib, abi
 from ._lib import Context, MrxError, load  # noqa: F401
-from .abi import GridStats, HiiBatchBox, HiiGlobals, TsGlobals, TsParams
+from .abi import GridStats, HiiBatchBox, HiiGlobals, TsGlobals, TsParams, TsTables
 
 try:  # torch is optional for host-memory use
     import torch
@@ -328,6 +328,25 @@ def compute_ts(ctx, params, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_
     out = {k: getattr(g, k) for k, _ in TsGlobals._fields_}
     out.update({"x_e_box": x_e_box, "x_e_box_prev": x_e_box_prev, "Tk_box": Tk_box, "TS_box": TS_box})
     return out
+
+
+def ts_last_shells(ctx, grid_dim, n_shells):
+    """The smoothed SFR shells of the last compute_ts call (max(filtered sfr, 0),
+    ComputeTs.c:379-503) as a host array [n_shells, N, N, N]."""
+    out = np.empty((n_shells, grid_dim, grid_dim, grid_dim), np.float32)
+    ctx.check(ctx.lib.mrx_ts_last_shells(ctx.ptr, grid_dim, n_shells, _ptr(out), 0), "ts_last_shells")
+    return out
+
+
+def ts_snapshot_tables(params, collapse_fraction, x_e_ave):
+    """The host tables of one spin-temperature snapshot (mrx_ts_snapshot_tables):
+    an mrx_ts_tables.  Host only; params.stored_fcoll[snapshot] receives
+    collapse_fraction."""
+    t = TsTables()
+    rc = load().mrx_ts_snapshot_tables(C.byref(params), float(collapse_fraction), float(x_e_ave), C.byref(t))
+    if rc != 0:
+        raise MrxError(rc, "ts_snapshot_tables")
+    return t
 
 
 def construct_baryon_grids(ctx, params, pos, type_, fesc_weighted_gsm, gross_stellar_mass=None, black_hole_mass=None,

@@ -17,7 +17,6 @@
 #include <string.h>
 
 #include "oracle.h"
-#include "../meraxes_amd/csrc/recomb_tables.h"
 
 /* constants of src/meraxes.h:21-44 and reionization.h:38 */
 #define ORC_PLANCK 6.6262e-27
@@ -274,17 +273,6 @@ static double orc_RtoM(const mrx_hii_params* p, double R)
   }
 }
 
-/* the recombination-rate table of the last cosmology asked for */
-static mrx_rr_tables* orc_rr(const mrx_hii_params* p)
-{
-  static mrx_rr_tables* t = NULL;
-  const double No = mrx_rr_No(p->hubble_h, p->omega_m, p->baryon_frac, p->y_he);
-  if (!t) t = (mrx_rr_tables*)calloc(1, sizeof *t);
-  if (t && (t->No != No || t->y_he != p->y_he))
-    if (mrx_rr_init(t, No, p->y_he) != 0) return NULL;
-  return t;
-}
-
 int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* deltax, const float* stars,
                          const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
                          float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out, float* max_ratio)
@@ -443,10 +431,12 @@ int orc_find_hii_bubbles_rec(int precision, const mrx_hii_params* p, const float
   }
 
   /* find_HII_bubbles.c:451-494: sequential double sums, mass weight from the input deltax */
-  const mrx_rr_tables* rr = recomb ? orc_rr(p) : NULL;
-  if (recomb && !rr) return MRX_ENOMEM;
+  if (recomb) {
+    const int rc = orc_rr_build(p->hubble_h, p->omega_m, p->baryon_frac, p->y_he);
+    if (rc != MRX_OK) return rc;
+  }
   const float zstep = (float)(p->prev_redshift - redshift);
-  const float fabs_dtdz = (float)fabs(mrx_cosmo_dtdz(p->omega_lambda, p->omega_m, p->hubble_h, (float)redshift) /
+  const float fabs_dtdz = (float)fabs(orc_dtdz(p->omega_lambda, p->omega_m, p->hubble_h, (float)redshift) /
                                       p->hubble_h);
   double vol_xH = 0.0, mass_xH = 0.0, mass_weight = 0.0, vol_J21 = 0.0;
   for (long r = 0; r < (long)N * N; r++)
@@ -461,7 +451,7 @@ int orc_find_hii_bubbles_rec(int precision, const mrx_hii_params* p, const float
       mass_weight += density_over_mean;
       if (recomb) { /* :472-477 */
         const float z_eff = (float)((1. + redshift) * pow(density_over_mean, 1.0 / 3.0) - 1);
-        const double dNrec = mrx_rr_rate(rr, z_eff, Gamma12[i_real]) * fabs_dtdz * zstep * (1. - cell_xH);
+        const double dNrec = orc_rr_rate(z_eff, Gamma12[i_real]) * fabs_dtdz * zstep * (1. - cell_xH);
         N_rec[i_padded] += dNrec;
       }
     }

@@ -65,13 +65,48 @@ int orc_find_hii_bubbles(int precision, const mrx_hii_params* p, const float* de
                          const float* weighted_sfr, const float* x_e, float* xH, float* r_bubble, float* J_21,
                          float* z_at_ionization, float* J_21_at_ionization, mrx_hii_globals* out, float* max_ratio);
 
-/* _ComputeTs restated (ts.c).  Same pointer contract as mrx_compute_ts (host
- * memory); shells_out, if not NULL, receives the num_filter_steps smoothed SFR
- * shells (float, shell-major, unpadded) of the inhomogeneous branch. */
-int orc_ts_export(const mrx_ts_params* p, double* out, int cap);
-int orc_compute_ts(const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
-                   const float* sfr_histories, float* x_e_box, float* x_e_box_prev, float* Tk_box, float* TS_box,
-                   mrx_ts_globals* out, float* shells_out);
+/* _ComputeTs restated (ts.c).  The snapshot's host tables (shell plan,
+ * frequency integrals, Ly-n sums, prefactors, spline coefficients) come from
+ * the oracle's own numpy restatement (oracle/ts_host.py) in an orc_ts_tab;
+ * everything per cell is restated in ts.c.  The kappa splines are natural
+ * cubic splines given by their knots and second-derivative coefficients c. */
+#define ORC_TS_MAX_STEPS 64
+#define ORC_TS_MAX_OPS 4096
+#define ORC_XHII 14
+typedef struct orc_spline {
+  int n;
+  double x[32], y[32], c[32];
+} orc_spline;
+typedef struct orc_ts_tab {
+  int nsteps, homogeneous, no_light, n_ops;
+  double zp, dzp, dt_dzp, const_zp_prefactor, growth_factor_zp, dgrowth_factor_dzp;
+  double No, N_b0, f_H, f_He;
+  double xe_recfast, T_recfast; /* xion_RECFAST((float)zp), T_RECFAST((float)zp): homogeneous branch */
+  double R[ORC_TS_MAX_STEPS], dzpp[ORC_TS_MAX_STEPS], dt_dzpp[ORC_TS_MAX_STEPS], zpp[ORC_TS_MAX_STEPS];
+  double zpp_pow[ORC_TS_MAX_STEPS], sum_lyn[ORC_TS_MAX_STEPS];
+  double freq_heat[ORC_XHII][ORC_TS_MAX_STEPS], freq_ion[ORC_XHII][ORC_TS_MAX_STEPS];
+  double freq_lya[ORC_XHII][ORC_TS_MAX_STEPS];
+  float xHII[ORC_XHII];
+  orc_spline kappa_HH, kappa_eH, kappa_pH;
+  /* SFR-history operations in plan order: 0 zero the first N*N*(N/2+1) floats,
+   * 1 scale them by weight, 2 load slot * (float)weight, 3 add it */
+  int op_shell[ORC_TS_MAX_OPS], op_kind[ORC_TS_MAX_OPS], op_slot[ORC_TS_MAX_OPS];
+  double op_weight[ORC_TS_MAX_OPS];
+} orc_ts_tab;
+
+/* The first shell's grid means (ComputeTs.c:225-257, :413-475): the mean
+ * collapse fraction at radius R (Mpc) and <x_e_box_prev>, summed in the
+ * reference's cell order. */
+int orc_ts_first_shell_means(const mrx_ts_params* p, double R, const float* deltax, const float* stars,
+                             const float* x_e_box_prev, double* collapse_fraction, double* x_e_ave);
+/* Same pointer contract as mrx_compute_ts (host memory); shells_out, if not
+ * NULL, receives the num_filter_steps smoothed SFR shells (float,
+ * shell-major, unpadded) of the inhomogeneous branch.  The globals reported
+ * as collapse_fraction / x_e_ave are the arguments (what the tables used). */
+int orc_compute_ts(const mrx_ts_params* p, const orc_ts_tab* t, double collapse_fraction, double x_e_ave,
+                   const float* deltax, const float* stars, const float* sfr, const float* sfr_histories,
+                   float* x_e_box, float* x_e_box_prev, float* Tk_box, float* TS_box, mrx_ts_globals* out,
+                   float* shells_out);
 
 /* ComputeBrightnessTemperatureBox restated (no peculiar velocities). */
 int orc_brightness_temp(const mrx_tb_params* p, const float* xH, const float* deltax, const float* TS_box,
@@ -97,11 +132,31 @@ int orc_rsd_los(const mrx_tb_vel_params* p, const float* xH, const float* vel, f
 
 /* _find_HII_bubbles with inhomogeneous recombinations (find_HII_bubbles.c:155-163,
  * :246-302, :325-404, :472-477): N_rec (padded) and Gamma12 (unpadded) in/out;
- * the recombination-rate table is recomb_tables.c (shared with the library). */
+ * the recombination-rate table is recomb.c's (built on first use per cosmology). */
 int orc_find_hii_bubbles_rec(int precision, const mrx_hii_params* p, const float* deltax, const float* stars,
                              const float* weighted_sfr, const float* x_e, float* N_rec, float* xH, float* r_bubble,
                              float* J_21, float* z_at_ionization, float* J_21_at_ionization, float* Gamma12,
                              mrx_hii_globals* out, float* max_ratio);
+
+/* recomb.c: recombinations.c restated (MHR00 PDF, self-shielding, rate table)
+ * with the oracle's own QAG-61 and natural spline.  orc_set_gk61 installs the
+ * 61-point rule (31 nonnegative nodes descending, their Kronrod weights, the
+ * 15 Gauss weights of the odd nodes) before any integral is taken. */
+void orc_set_gk61(const double* xgk, const double* wgk, const double* wg);
+double orc_qag61(double (*f)(double, void*), void* par, double a, double b, double epsrel, int limit,
+                 double* abserr);
+int orc_rr_build(double hubble_h, double omega_m, double baryon_frac, double y_he);
+double orc_rr_rate(double z_eff, double gamma12_bg);
+double orc_rr_A(double z);
+double orc_rr_C(double z);
+double orc_rr_beta(double z);
+double orc_rr_Gamma_SS(double Gamma_bg, double Delta, double T_4, double z);
+double orc_rr_neutral_fraction(double y_he, double density, double T4, double gamma12);
+double orc_rr_recombination_rate(double z_eff, double gamma12_bg, double T4);
+double orc_rr_A_aux_integral(double z);
+double orc_rr_table(int z_ct, int g);
+double orc_rr_lnG(int g);
+double orc_dtdz(double omega_l, double omega_m, double hubble_h, float z);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,8 @@ import numpy as np
 
 from meraxes_amd.abi import GridParams, HiiGlobals, HiiParams, TbParams, TbVelParams, TsGlobals, TsParams, fptr
 
+from . import ts_host
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
@@ -45,14 +47,20 @@ def lib():
         L.orc_window_f32.restype = f
         L.orc_radius_ladder.argtypes = [C.POINTER(HiiParams), vp, i]
         L.orc_brightness_temp.argtypes = [C.POINTER(TbParams), vp, vp, vp, vp, C.POINTER(C.c_double)]
-        L.orc_compute_ts.argtypes = [C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals), vp]
-        L.orc_ts_export.argtypes = [C.POINTER(TsParams), vp, i]
+        L.orc_compute_ts.argtypes = [C.POINTER(TsParams), vp, d, d] + [vp] * 8 + [C.POINTER(TsGlobals), vp]
+        L.orc_ts_first_shell_means.argtypes = [C.POINTER(TsParams), d, vp, vp, vp, vp, vp]
         L.orc_construct_baryon_grids.argtypes = [C.POINTER(GridParams), C.c_long] + [vp] * 11
         L.orc_brightness_temp_vel.argtypes = [C.POINTER(TbVelParams)] + [vp] * 5 + [C.POINTER(C.c_double)]
         L.orc_rsd_los.argtypes = [C.POINTER(TbVelParams), vp, vp, vp]
         L.orc_find_hii_bubbles_rec.argtypes = [i, C.POINTER(HiiParams)] + [vp] * 11 + [C.POINTER(HiiGlobals), vp]
         L.orc_set_threads.argtypes = [i]
         L.orc_get_threads.restype = i
+        L.orc_set_gk61.argtypes = [vp, vp, vp]
+        L.orc_qag61.restype = d
+        L.orc_qag61.argtypes = [vp, vp, d, d, d, i, vp]
+        # the 61-point Gauss-Kronrod rule of oracle/ts_host.py, for recomb.c's QAG
+        _rule = [np.ascontiguousarray(a, np.float64) for a in (ts_host.XGK, ts_host.WGK, ts_host.WG)]
+        L.orc_set_gk61(*[a.ctypes.data_as(vp) for a in _rule])
         _lib = L
     return _lib
 
@@ -206,23 +214,20 @@ def _find_hii_bubbles_rec(p, deltax, stars, wsfr, x_e, z_at_ion, J21_at_ion, pre
     }
 
 
-def rr_tables_lib():
-    """The oracle library's recombination-table functions (recomb_tables.c), for the tests."""
+def rr_lib():
+    """The oracle's recombination functions (oracle/recomb.c), for the tests."""
     L = lib()
     d = C.c_double
-    for name, args in (("mrx_rr_No", [d, d, d, d]), ("mrx_rr_neutral_fraction", [d, d, d, d, C.c_int]),
-                       ("mrx_rr_Gamma_SS", [d, d, d, d]), ("mrx_cosmo_dtdz", [d, d, d, C.c_float])):
+    sig = {"orc_rr_rate": [d, d], "orc_rr_A": [d], "orc_rr_C": [d], "orc_rr_beta": [d],
+           "orc_rr_Gamma_SS": [d, d, d, d], "orc_rr_neutral_fraction": [d, d, d, d],
+           "orc_rr_recombination_rate": [d, d, d], "orc_rr_A_aux_integral": [d], "orc_rr_table": [C.c_int, C.c_int],
+           "orc_rr_lnG": [C.c_int], "orc_dtdz": [d, d, d, C.c_float]}
+    for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = d
-    for name, args in (("mrx_rr_A_MHR", [C.c_void_p, d]), ("mrx_rr_C_MHR", [C.c_void_p, d]),
-                       ("mrx_rr_beta_MHR", [C.c_void_p, d]), ("mrx_rr_rate", [C.c_void_p, C.c_float, d]),
-                       ("mrx_rr_recombination_rate", [C.c_void_p, d, d, d, C.c_int])):
-        f = getattr(L, name)
-        f.argtypes = args
-        f.restype = d
-    L.mrx_rr_init.argtypes = [C.c_void_p, d, d]
-    L.mrx_rr_init.restype = C.c_int
+    L.orc_rr_build.argtypes = [d, d, d, d]
+    L.orc_rr_build.restype = C.c_int
     return L
 
 
@@ -255,6 +260,86 @@ def rsd_los(p, xH, vel_converted, delta_T):
     return out
 
 
+class _Spline(C.Structure):
+    _fields_ = [("n", C.c_int), ("x", C.c_double * 32), ("y", C.c_double * 32), ("c", C.c_double * 32)]
+
+
+_MS, _MO, _NX = 64, 4096, 14
+
+
+class TsTab(C.Structure):
+    """ctypes mirror of orc_ts_tab (oracle.h)."""
+    _fields_ = [("nsteps", C.c_int), ("homogeneous", C.c_int), ("no_light", C.c_int), ("n_ops", C.c_int),
+                ("zp", C.c_double), ("dzp", C.c_double), ("dt_dzp", C.c_double), ("const_zp_prefactor", C.c_double),
+                ("growth_factor_zp", C.c_double), ("dgrowth_factor_dzp", C.c_double),
+                ("No", C.c_double), ("N_b0", C.c_double), ("f_H", C.c_double), ("f_He", C.c_double),
+                ("xe_recfast", C.c_double), ("T_recfast", C.c_double),
+                ("R", C.c_double * _MS), ("dzpp", C.c_double * _MS), ("dt_dzpp", C.c_double * _MS),
+                ("zpp", C.c_double * _MS), ("zpp_pow", C.c_double * _MS), ("sum_lyn", C.c_double * _MS),
+                ("freq_heat", (C.c_double * _MS) * _NX), ("freq_ion", (C.c_double * _MS) * _NX),
+                ("freq_lya", (C.c_double * _MS) * _NX), ("xHII", C.c_float * _NX),
+                ("kappa_HH", _Spline), ("kappa_eH", _Spline), ("kappa_pH", _Spline),
+                ("op_shell", C.c_int * _MO), ("op_kind", C.c_int * _MO), ("op_slot", C.c_int * _MO),
+                ("op_weight", C.c_double * _MO)]
+
+
+def _fill_spline(dst, sp):
+    dst.n = len(sp.x)
+    for i in range(dst.n):
+        dst.x[i], dst.y[i], dst.c[i] = sp.x[i], sp.y[i], sp.c[i]
+
+
+def ts_tables(p, deltax=None, stars=None, x_e_box_prev=None):
+    """The snapshot's host tables from the oracle's restatement (oracle/ts_host.py).
+
+    The two grid means the tables depend on (ComputeTs.c:413-475) are taken
+    from the grids (or from p.fixed_* when p.fixed_globals is set); like the
+    reference, stored_fcoll[snapshot] receives the collapse fraction.
+    Returns (TsSnapshot, orc_ts_tab, collapse_fraction, x_e_ave)."""
+    h = ts_host.host(p)
+    s = h.plan()
+    cf = xe = 0.0
+    if deltax is not None:
+        R0 = ts_host.L_FACTOR * (p.box_size / p.hubble_h) / float(np.float32(p.grid_dim))
+        cfv, xev = C.c_double(), C.c_double()
+        args = [np.ascontiguousarray(a, dtype=np.float32) for a in (deltax, stars, x_e_box_prev)]
+        _chk(lib().orc_ts_first_shell_means(C.byref(p), R0, *[fptr(a) for a in args], C.byref(cfv), C.byref(xev)),
+             "ts_first_shell_means")
+        cf, xe = cfv.value, (0.0 if s.homogeneous else xev.value)
+    if p.fixed_globals and not s.homogeneous:
+        cf, xe = p.fixed_collapse_fraction, p.fixed_x_e_ave
+    h.tables(s, cf, xe, p.stored_fcoll)
+    if not s.homogeneous and p.stored_fcoll:
+        p.stored_fcoll[p.snapshot] = cf
+    t = TsTab()
+    t.nsteps, t.homogeneous = s.nsteps, int(s.homogeneous)
+    t.zp, t.dzp, t.dt_dzp = s.zp, s.dzp, s.dt_dzp
+    t.No, t.N_b0, t.f_H, t.f_He = h.No, h.N_b0, h.f_H, h.f_He
+    for k in range(ts_host.NXHII):
+        t.xHII[k] = float(ts_host.XHII[k])
+    _fill_spline(t.kappa_HH, h.d.kappa_HH)
+    _fill_spline(t.kappa_eH, h.d.kappa_eH)
+    _fill_spline(t.kappa_pH, h.d.kappa_pH)
+    if s.homogeneous:
+        t.xe_recfast, t.T_recfast = s.xe_recfast, s.T_recfast
+        return s, t, cf, xe
+    if len(s.ops) > _MO:
+        raise ValueError("more SFR-history operations than the oracle holds")
+    t.no_light = s.no_light
+    t.const_zp_prefactor, t.growth_factor_zp, t.dgrowth_factor_dzp = (s.const_zp_prefactor, s.growth_factor_zp,
+                                                                      s.dgrowth_factor_dzp)
+    for c in range(s.nsteps):
+        t.R[c], t.dzpp[c], t.dt_dzpp[c], t.zpp[c] = s.R[c], s.dzpp[c], s.dt_dzpp[c], s.zpp[c]
+        t.zpp_pow[c], t.sum_lyn[c] = s.zpp_pow[c], s.sum_lyn[c]
+        for k in range(ts_host.NXHII):
+            t.freq_heat[k][c], t.freq_ion[k][c], t.freq_lya[k][c] = (s.freq_heat[k, c], s.freq_ion[k, c],
+                                                                     s.freq_lya[k, c])
+    t.n_ops = len(s.ops)
+    for k, (c, kind, slot, w) in enumerate(s.ops):
+        t.op_shell[k], t.op_kind[k], t.op_slot[k], t.op_weight[k] = c, kind, slot, w
+    return s, t, cf, xe
+
+
 def compute_ts(p, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box, want_shells=False):
     """Runs the restated _ComputeTs; x_e_box_prev / Tk_box are copied, not modified."""
     N = p.grid_dim
@@ -263,49 +348,48 @@ def compute_ts(p, deltax, stars, sfr, sfr_histories, x_e_box_prev, Tk_box, want_
     xe = np.empty_like(xe_prev)
     TS = np.empty((N, N, N), np.float32)
     shells = np.empty((p.num_filter_steps, N, N, N), np.float32) if want_shells else None
+    _, t, cf, xav = ts_tables(p, deltax, stars, xe_prev)
     g = TsGlobals()
     args = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None
             for a in (deltax, stars, sfr, sfr_histories)]
-    _chk(lib().orc_compute_ts(C.byref(p), *[fptr(a) for a in args], fptr(xe), fptr(xe_prev), fptr(Tk), fptr(TS),
-                              C.byref(g), fptr(shells)), "compute_ts")
+    _chk(lib().orc_compute_ts(C.byref(p), C.byref(t), cf, xav, *[fptr(a) for a in args], fptr(xe), fptr(xe_prev),
+                              fptr(Tk), fptr(TS), C.byref(g), fptr(shells)), "compute_ts")
     out = {k: getattr(g, k) for k, _ in TsGlobals._fields_}
     out.update({"x_e_box": xe, "x_e_box_prev": xe_prev, "Tk_box": Tk, "TS_box": TS, "shells": shells})
     return out
 
 
+def _spline_dict(sp):
+    """Knots and per-interval coefficients (y + dx (b + dx (c + dx d))) of a natural spline."""
+    x, y, c = np.asarray(sp.x), np.asarray(sp.y), np.asarray(sp.c)
+    h = np.diff(x)
+    b = np.zeros_like(x)
+    d = np.zeros_like(x)
+    b[:-1] = (y[1:] - y[:-1]) / h - h * (c[1:] + 2.0 * c[:-1]) / 3.0
+    d[:-1] = (c[1:] - c[:-1]) / (3.0 * h)
+    return {"x": x.copy(), "y": y.copy(), "b": b, "c": c.copy(), "d": d}
+
+
 def ts_export(p):
     """Constants of the inhomogeneous spin-temperature step (p.fixed_globals must be set)."""
-    buf = np.zeros(200000, np.float64)
-    n = lib().orc_ts_export(C.byref(p), buf.ctypes.data_as(C.c_void_p), len(buf))
-    _chk(0 if n > 0 else n, "ts_export")
-    o = buf[:n]
-    ns = int(o[0])
-    out = {"nsteps": ns, "no_light": int(o[1]), "zp": o[2], "dzp": o[3], "dt_dzp": o[4],
-           "const_zp_prefactor": o[5], "growth": o[6], "dgrowth": o[7], "N_b0": o[8], "No": o[9], "f_H": o[10],
-           "f_He": o[11], "H": o[12], "hubble_h": o[13]}
-    nop = int(o[14])
-    k = 16
+    if not p.fixed_globals:
+        raise ValueError("ts_export needs p.fixed_globals")
+    s, _, _, _ = ts_tables(p)
+    if s.homogeneous:
+        raise ValueError("ts_export: homogeneous snapshot")
+    h = ts_host.host(p)
+    out = {"nsteps": s.nsteps, "no_light": s.no_light, "zp": s.zp, "dzp": s.dzp, "dt_dzp": s.dt_dzp,
+           "const_zp_prefactor": s.const_zp_prefactor, "growth": s.growth_factor_zp,
+           "dgrowth": s.dgrowth_factor_dzp, "N_b0": h.N_b0, "No": h.No, "f_H": h.f_H, "f_He": h.f_He,
+           "H": h.hubble(s.zp), "hubble_h": p.hubble_h}
     for name in ("R", "dt_dzpp", "dzpp", "zpp", "zpp_pow", "sum_lyn"):
-        out[name] = o[k:k + ns].copy()
-        k += ns
+        out[name] = np.array(getattr(s, name), np.float64)
     for name in ("freq_heat", "freq_ion", "freq_lya"):
-        out[name] = o[k:k + 14 * ns].reshape(14, ns).copy()
-        k += 14 * ns
-    out["XH"] = o[k:k + 14].astype(np.float32)
-    k += 14
-    for name in ("kHH", "keH", "kpH"):
-        m = int(o[k])
-        k += 1
-        sp = {}
-        for f in ("x", "y", "b", "c", "d"):
-            sp[f] = o[k:k + m].copy()
-            k += 27
-        out[name] = sp
-    ops = []
-    for _ in range(nop):
-        ops.append((int(o[k]), int(o[k + 1]), int(o[k + 2]), float(o[k + 3])))
-        k += 4
-    out["ops"] = ops
+        out[name] = np.array(getattr(s, name), np.float64)
+    out["XH"] = ts_host.XHII.copy()
+    out["kHH"], out["keH"], out["kpH"] = (_spline_dict(h.d.kappa_HH), _spline_dict(h.d.kappa_eH),
+                                          _spline_dict(h.d.kappa_pH))
+    out["ops"] = list(s.ops)
     return out
 
 

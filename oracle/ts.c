@@ -4,15 +4,15 @@
  *
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
- * The per-snapshot scalar tables (shell radii, SFR-history weights, frequency
- * integrals, sum_lyn, prefactors) come from meraxes_amd/csrc/ts_tables.c,
- * which this oracle shares with the library and which the tests pin against
- * scipy.  Everything per cell is restated here on its own:
+ * The per-snapshot host tables (shell radii, SFR-history operations,
+ * frequency integrals, sum_lyn, prefactors, spline coefficients) arrive in an
+ * orc_ts_tab built by the oracle's own restatement oracle/ts_host.py; this
+ * file shares no code with the library.  Everything per cell is restated here:
  *   shell loop: ComputeTs.c:280-503 (working SFR grid ops, R2C, /N^3 in float,
  *     filter for R_ct > 0 with R in Mpc and BoxSize in Mpc/h, C2R, max(v,0));
- *   collapse fraction and <x_e> of the first shell: :413-475;
+ *   collapse fraction and <x_e> of the first shell: :225-257, :413-475;
  *   per-cell update: :743-900 with evolveInt (XRayHeatingFunctions.c:1241-1435),
- *     get_Ts and helpers (:1438-1882);
+ *     get_Ts and helpers (:1438-1882), kappa splines (:1602-1845);
  *   homogeneous branch: :189-262; averages: :940-985.
  * Build with -ffp-contract=off.
  */
@@ -20,7 +20,6 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "../meraxes_amd/csrc/ts_tables.h"
 #include "oracle.h"
 
 #define O_TCMB 2.728
@@ -38,25 +37,42 @@
 
 /* ------------------------------------------------------ per-cell physics */
 
-static double o_hubble(const mrx_ts_data* d, float z)
+static double o_hubble(const mrx_ts_params* p, float z)
 {
-  return O_HUBBLE * d->hubble_h * sqrt(d->omega_m * pow(1 + z, 3) + d->omega_r * pow(1 + z, 4) + d->omega_l);
+  return O_HUBBLE * p->hubble_h * sqrt(p->omega_m * pow(1 + z, 3) + p->omega_r * pow(1 + z, 4) + p->omega_lambda);
 }
 
-static double o_kappa_HH(const mrx_ts_data* d, double TK)
+/* gsl_spline_eval of a natural cubic spline (interval by bisection) */
+static double o_spline(const orc_spline* s, double xv)
 {
-  const mrx_spline* s = &d->kappa_HH;
+  int lo = 0, hi = s->n - 1;
+  while (hi > lo + 1) {
+    const int i = (hi + lo) / 2;
+    if (s->x[i] > xv)
+      hi = i;
+    else
+      lo = i;
+  }
+  const double h = s->x[lo + 1] - s->x[lo], dx = xv - s->x[lo], dy = s->y[lo + 1] - s->y[lo];
+  const double b = dy / h - h * (s->c[lo + 1] + 2.0 * s->c[lo]) / 3.0;
+  const double d = (s->c[lo + 1] - s->c[lo]) / (3.0 * h);
+  return s->y[lo] + dx * (b + dx * (s->c[lo] + dx * d));
+}
+
+static double o_kappa_HH(const orc_ts_tab* t, double TK)
+{
+  const orc_spline* s = &t->kappa_HH;
   double ans;
   if (log(TK) < s->x[0])
     ans = s->y[0];
   else if (log(TK) > s->x[s->n - 1])
     ans = log(exp(s->y[s->n - 1]) * pow(TK / exp(s->x[s->n - 1]), 0.381));
   else
-    ans = mrx_spline_eval(s, log(TK));
+    ans = o_spline(s, log(TK));
   return exp(ans);
 }
 
-static double o_kappa_file(const mrx_spline* s, double T)
+static double o_kappa_file(const orc_spline* s, double T)
 {
   const int n = s->n;
   double ans;
@@ -66,11 +82,28 @@ static double o_kappa_file(const mrx_spline* s, double T)
   else if (T > s->x[n - 1])
     ans = s->y[n - 1] + ((s->y[n - 1] - s->y[n - 2]) / (s->x[n - 1] - s->x[n - 2]) * (T - s->x[n - 1]));
   else
-    ans = mrx_spline_eval(s, T);
+    ans = o_spline(s, T);
   return exp(ans);
 }
 
-static double o_xcoll(const mrx_ts_data* d, double z, double TK, double delta, double xe)
+/* alpha_A, XRayHeatingFunctions.c:1450-1458 */
+static double o_alpha_A(double T)
+{
+  const double logT = log(T / (double)1.1604505e4);
+  return exp(-28.6130338 - 0.72411256 * logT - 2.02604473e-2 * pow(logT, 2) - 2.38086188e-3 * pow(logT, 3) -
+             3.21260521e-4 * pow(logT, 4) - 1.42150291e-5 * pow(logT, 5) + 4.98910892e-6 * pow(logT, 6) +
+             5.75561414e-7 * pow(logT, 7) - 1.85676704e-8 * pow(logT, 8) - 3.07113524e-9 * pow(logT, 9));
+}
+
+/* locate_xHII_index, XRayHeatingFunctions.c:1210-1220 */
+static int o_locate_xHII(const orc_ts_tab* t, float x)
+{
+  int m = ORC_XHII - 1;
+  while (x < t->xHII[m]) m--;
+  return m;
+}
+
+static double o_xcoll(const orc_ts_tab* d, double z, double TK, double delta, double xe)
 {
   const double Trad = O_TCMB * (1.0 + z);
   const double nH = (1.0 - xe) * d->No * pow(1.0 + z, 3.0) * (1.0 + delta);
@@ -82,10 +115,10 @@ static double o_xcoll(const mrx_ts_data* d, double z, double TK, double delta, d
   return x_HI + x_e + x_p;
 }
 
-static double o_xalpha_tilde(const mrx_ts_data* d, double z, double Jalpha, double TK, double TS, double delta,
-                             double xe)
+static double o_xalpha_tilde(const mrx_ts_params* p, const orc_ts_tab* d, double z, double Jalpha, double TK,
+                             double TS, double delta, double xe)
 {
-  const double tgp = 1.342881e-7 / o_hubble(d, (float)z) * d->No * pow(1 + z, 3) * (1.0 + delta) * (1.0 - xe);
+  const double tgp = 1.342881e-7 / o_hubble(p, (float)z) * d->No * pow(1 + z, 3) * (1.0 + delta) * (1.0 - xe);
   const double xi = pow(1.0e-7 * tgp / TK / TK, 1.0 / 3.0);
   double S = 1.0 - 0.0631789 / TK + 0.115995 / TK / TK - 0.401403 / TS / TK;
   S += 0.336463 / TS / TK / TK;
@@ -99,7 +132,8 @@ static double o_Tc_eff(double TK, double TS)
   return 1.0 / ans;
 }
 
-static float o_get_Ts(const mrx_ts_data* d, float z, float delta, float TK, float xe, float Jalpha, float* xa_out)
+static float o_get_Ts(const mrx_ts_params* p, const orc_ts_tab* d, float z, float delta, float TK, float xe,
+                      float Jalpha, float* xa_out)
 {
   const double Trad = O_TCMB * (1.0 + z);
   const double xc = o_xcoll(d, z, TK, delta, xe);
@@ -109,7 +143,7 @@ static float o_get_Ts(const mrx_ts_data* d, float z, float delta, float TK, floa
     TS = Trad;
     while (fabs(TS - TSold) / TS > 1.0e-3) {
       TSold = TS;
-      xa = o_xalpha_tilde(d, z, Jalpha, TK, TS, delta, xe);
+      xa = o_xalpha_tilde(p, d, z, Jalpha, TK, TS, delta, xe);
       const double Tceff = o_Tc_eff(TK, TS);
       const double TSinv = (1.0 / Trad + xa / Tceff + xc / TK) / (1.0 + xa + xc);
       TS = 1.0 / TSinv;
@@ -124,22 +158,22 @@ static float o_get_Ts(const mrx_ts_data* d, float z, float delta, float TK, floa
   return (float)TS;
 }
 
-static double o_dT_comp(const mrx_ts_data* d, double z, double TK, double xe)
+static double o_dT_comp(const mrx_ts_params* p, const orc_ts_tab* d, double z, double TK, double xe)
 {
   const double Trad = O_TCMB * (1.0 + z);
-  double ans = (-1.51e-4) * (xe / (1.0 + xe + d->f_He)) / (o_hubble(d, (float)z) / (O_HUBBLE * d->hubble_h)) /
-               d->hubble_h * pow(Trad, 4.0) / (1.0 + z);
+  double ans = (-1.51e-4) * (xe / (1.0 + xe + d->f_He)) / (o_hubble(p, (float)z) / (O_HUBBLE * p->hubble_h)) /
+               p->hubble_h * pow(Trad, 4.0) / (1.0 + z);
   ans *= Trad - TK;
   return ans;
 }
 
 /* evolveInt without minihalos; deriv[0..4] as the reference fills them */
-static void o_evolve(const mrx_ts_data* d, const mrx_ts_snap* s, float zp, float curr_delNL0, const double* SFR,
+static void o_evolve(const mrx_ts_params* p, const orc_ts_tab* s, float zp, float curr_delNL0, const double* SFR,
                      const double* fh, const double* fi, const double* fl, const double* y, double* deriv)
 {
   const double Conversion_factor = (O_SPEED_OF_LIGHT / (4. * M_PI)) / (O_PROTONMASS / O_SOLAR_MASS);
   const double x_e = y[0], T = y[1];
-  const double n_b = d->N_b0 * pow(1 + zp, 3) * (1 + curr_delNL0);
+  const double n_b = s->N_b0 * pow(1 + zp, 3) * (1 + curr_delNL0);
   double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
   if (!s->no_light) {
     for (int ct = 0; ct < s->nsteps; ct++) {
@@ -155,14 +189,14 @@ static void o_evolve(const mrx_ts_data* d, const mrx_ts_snap* s, float zp, float
     dxlya *= s->const_zp_prefactor * n_b;
     dstarlya *= Conversion_factor;
   }
-  const double sink = mrx_ts_alpha_A(T) * O_CLUMPING * x_e * x_e * d->f_H * n_b;
+  const double sink = o_alpha_A(T) * O_CLUMPING * x_e * x_e * s->f_H * n_b;
   const double dxe_dzp = s->dt_dzp * (dxion - sink);
   deriv[0] = dxe_dzp;
   double dadia = 3 / (1.0 + zp);
   if (fabs(curr_delNL0) > O_FRACT_FLOAT_ERR) dadia += s->dgrowth_factor_dzp / (1.0 / curr_delNL0 + s->growth_factor_zp);
   dadia *= (2.0 / 3.0) * T;
   const double dspec = -dxe_dzp * T / (1 + x_e);
-  const double dcomp = o_dT_comp(d, zp, T, x_e);
+  const double dcomp = o_dT_comp(p, s, zp, T, x_e);
   const double dxheat_dzp = dxheat * s->dt_dzp * 2.0 / 3.0 / O_BOLTZMANN / (1.0 + x_e);
   deriv[1] = dxheat_dzp + dcomp + dspec + dadia;
   deriv[2] = dxlya + dstarlya;
@@ -178,82 +212,99 @@ static double o_RtoM(const mrx_ts_params* p, double R)
   return pow(2 * M_PI, 1.5) * p->omega_m * p->rho_crit * pow(R, 3);
 }
 
-int orc_compute_ts(const mrx_ts_params* p, const float* deltax, const float* stars, const float* sfr,
-                   const float* hist, float* x_e_box, float* x_e_box_prev, float* Tk_box, float* TS_box,
-                   mrx_ts_globals* out, float* shells_out)
+
+/* collapse fraction in one cell at radius R (ComputeTs.c:236-242, :447-451) */
+static double o_cell_fcoll(const mrx_ts_params* p, double R, double pixel_volume, float stars, float deltax)
+{
+  const double h = p->hubble_h;
+  const double dom = 1.0 + deltax;
+  double c = stars / (o_RtoM(p, R) * h * h * dom) * (4.0 / 3.0) * M_PI * pow(R, 3.0) / pixel_volume;
+  return c > 1.0 ? 1.0 : c;
+}
+
+int orc_ts_first_shell_means(const mrx_ts_params* p, double R, const float* deltax, const float* stars,
+                             const float* x_e_box_prev, double* collapse_fraction, double* x_e_ave)
 {
   const int N = p->grid_dim;
-  if (N < 2 || (N & 1) || !deltax || !stars || !sfr || !x_e_box || !x_e_box_prev || !Tk_box || !TS_box)
+  if (N < 2 || (N & 1) || !deltax || !stars || !x_e_box_prev) return MRX_EINVAL;
+  const int PR = 2 * (N / 2 + 1);
+  const double pixel_volume = pow(p->box_size / p->hubble_h / (double)N, 3);
+  const double total_n_cells = pow((double)N, 3);
+  double cf = 0.0, xe = 0.0;
+  for (long r = 0; r < (long)N * N; r++)
+    for (int z = 0; z < N; z++) {
+      const long ip = r * PR + z;
+      cf += o_cell_fcoll(p, R, pixel_volume, stars[ip], deltax[ip]);
+      xe += x_e_box_prev[ip];
+    }
+  if (collapse_fraction) *collapse_fraction = cf / total_n_cells;
+  if (x_e_ave) *x_e_ave = xe / total_n_cells;
+  return MRX_OK;
+}
+
+int orc_compute_ts(const mrx_ts_params* p, const orc_ts_tab* t, double collapse_fraction, double x_e_ave,
+                   const float* deltax, const float* stars, const float* sfr, const float* hist, float* x_e_box,
+                   float* x_e_box_prev, float* Tk_box, float* TS_box, mrx_ts_globals* out, float* shells_out)
+{
+  const int N = p->grid_dim;
+  if (N < 2 || (N & 1) || !t || !deltax || !stars || !sfr || !x_e_box || !x_e_box_prev || !Tk_box || !TS_box)
     return MRX_EINVAL;
-  mrx_ts_data* d = (mrx_ts_data*)malloc(sizeof(mrx_ts_data));
-  mrx_ts_snap* s = (mrx_ts_snap*)malloc(sizeof(mrx_ts_snap));
-  int rc = mrx_ts_data_init(d, p);
-  if (rc == MRX_OK) rc = mrx_ts_plan(s, d, p);
-  if (rc != MRX_OK) {
-    free(d);
-    free(s);
-    return rc;
-  }
-  if (!s->homogeneous && s->max_slot >= 0 && (!hist || s->max_slot >= p->n_history)) {
-    free(d);
-    free(s);
-    return MRX_EINVAL;
-  }
+  if (t->nsteps < 1 || t->nsteps > ORC_TS_MAX_STEPS || t->n_ops < 0 || t->n_ops > ORC_TS_MAX_OPS) return MRX_EINVAL;
+  for (int k = 0; k < t->n_ops; k++)
+    if (t->op_kind[k] >= 2 && (!hist || t->op_slot[k] < 0 || t->op_slot[k] >= p->n_history)) return MRX_EINVAL;
   const int PR = 2 * (N / 2 + 1);
   const long nreal = (long)N * N * N;
   const size_t npad = (size_t)N * N * PR;
   const size_t half = (size_t)N * N * (N / 2 + 1); /* slab_n_complex floats */
-  const double h = p->hubble_h;
-  const double box_size = p->box_size / h; /* Mpc */
+  const double box_size = p->box_size / p->hubble_h; /* Mpc */
   const double pixel_volume = pow(box_size / (double)N, 3);
   const double total_n_cells = pow((double)N, 3);
-  const float zp = (float)s->zp;
+  const float zp = (float)t->zp;
   const double unit_fac = (p->unit_mass_g / p->unit_time_s);
   const double len_fac = pow(p->unit_length_cm, -3.);
   double J_alpha_ave = 0, xalpha_ave = 0, Xheat_ave = 0, Xion_ave = 0;
-  double collapse_fraction = 0.0, x_e_ave = 0.0;
 
-  if (s->homogeneous) {
-    const float xe0 = (float)mrx_ts_xion_RECFAST(d, zp);
-    const float T0 = (float)mrx_ts_T_RECFAST(d, zp);
+  if (t->homogeneous) {
+    const float xe0 = (float)t->xe_recfast;
+    const float T0 = (float)t->T_recfast;
     for (long r = 0; r < (long)N * N; r++)
       for (int z = 0; z < N; z++) {
         const long ir = r * N + z, ip = r * PR + z;
         float xa;
         x_e_box_prev[ip] = xe0;
         Tk_box[ir] = T0;
-        TS_box[ir] = o_get_Ts(d, zp, deltax[ip], Tk_box[ir], x_e_box_prev[ip], 0, &xa);
+        TS_box[ir] = o_get_Ts(p, t, zp, deltax[ip], Tk_box[ir], x_e_box_prev[ip], 0, &xa);
       }
-    const double R = O_L_FACTOR * box_size / (float)N;
-    for (long r = 0; r < (long)N * N; r++)
-      for (int z = 0; z < N; z++) {
-        const long ip = r * PR + z;
-        const double dom = 1.0 + deltax[ip];
-        double c = stars[ip] / (o_RtoM(p, R) * h * h * dom) * (4.0 / 3.0) * M_PI * pow(R, 3.0) / pixel_volume;
-        if (c > 1.0) c = 1.0;
-        collapse_fraction += c;
-      }
-    collapse_fraction /= total_n_cells;
   } else {
     float* work = (float*)malloc(sizeof(float) * npad);
     float* filt = (float*)malloc(sizeof(float) * npad);
     float* spec = (float*)malloc(sizeof(float) * npad);
-    float* shells = (float*)malloc(sizeof(float) * (size_t)s->nsteps * nreal);
+    float* shells = (float*)malloc(sizeof(float) * (size_t)t->nsteps * nreal);
+    if (!work || !filt || !spec || !shells) {
+      free(work);
+      free(filt);
+      free(spec);
+      free(shells);
+      return MRX_ENOMEM;
+    }
     memcpy(work, sfr, sizeof(float) * npad);
-    for (int ct = 0; ct < s->nsteps; ct++) {
-      for (int k = 0; k < s->n_ops[ct]; k++) {
-        const mrx_sfr_op* o = &s->ops[ct][k];
-        if (o->kind == MRX_SFR_ZERO_HALF) {
+    int k = 0;
+    for (int ct = 0; ct < t->nsteps; ct++) {
+      for (; k < t->n_ops && t->op_shell[k] == ct; k++) {
+        const int kind = t->op_kind[k];
+        if (kind == 0) {
           for (size_t i = 0; i < half; i++) work[i] = 0;
-        } else if (o->kind == MRX_SFR_SCALE_HALF) {
-          for (size_t i = 0; i < half; i++) work[i] *= o->weight;
+        } else if (kind == 1) {
+          for (size_t i = 0; i < half; i++) work[i] *= t->op_weight[k];
         } else {
-          const float w = (float)o->weight;
-          const float* hs = hist + (size_t)o->slot * npad;
+          /* load_reion_sfr_grids (reionization.c:1756-1784): float weight,
+           * slot read with INDEX_REAL, written with INDEX_PADDED */
+          const float w = (float)t->op_weight[k];
+          const float* hs = hist + (size_t)t->op_slot[k] * npad;
           for (long r = 0; r < (long)N * N; r++)
             for (int z = 0; z < N; z++) {
-              const float v = hs[r * N + z] * w; /* INDEX_REAL read of a padded slot */
-              if (o->kind == MRX_SFR_LOAD)
+              const float v = hs[r * N + z] * w;
+              if (kind == 2)
                 work[r * PR + z] = v;
               else
                 work[r * PR + z] += v;
@@ -263,78 +314,54 @@ int orc_compute_ts(const mrx_ts_params* p, const float* deltax, const float* sta
       orc_r2c3d_f32(N, work, spec);
       const float fN3 = (float)total_n_cells;
       for (size_t i = 0; i < npad; i++) spec[i] /= fN3;
-      if (ct > 0) orc_filter_f32(spec, 0, N, N, (float)s->R[ct], p->heating_filter_type, p->box_size);
+      if (ct > 0) orc_filter_f32(spec, 0, N, N, (float)t->R[ct], p->heating_filter_type, p->box_size);
       orc_c2r3d_f32(N, spec, filt);
       for (long r = 0; r < (long)N * N; r++)
-        for (int z = 0; z < N; z++) {
-          const float v = fmaxf(filt[r * PR + z], 0.0);
-          shells[(size_t)ct * nreal + r * N + z] = v;
-        }
-      if (ct == 0) {
-        const double R = s->R[0];
-        for (long r = 0; r < (long)N * N; r++)
-          for (int z = 0; z < N; z++) {
-            const long ip = r * PR + z;
-            const double dom = 1.0 + deltax[ip];
-            double c = stars[ip] / (o_RtoM(p, R) * h * h * dom) * (4.0 / 3.0) * M_PI * pow(R, 3.0) / pixel_volume;
-            if (c > 1.0) c = 1.0;
-            collapse_fraction += c;
-            x_e_ave += x_e_box_prev[ip];
-          }
-        collapse_fraction /= total_n_cells;
-        x_e_ave /= total_n_cells;
-      }
+        for (int z = 0; z < N; z++) shells[(size_t)ct * nreal + r * N + z] = fmaxf(filt[r * PR + z], 0.0);
     }
-    if (p->fixed_globals) {
-      collapse_fraction = p->fixed_collapse_fraction;
-      x_e_ave = p->fixed_x_e_ave;
-    }
-    s->collapse_fraction = collapse_fraction;
-    s->x_e_ave = x_e_ave;
-    mrx_ts_build_tables(s, d, p);
 
-    const float* XH = d->xi_XHII;
+    const float* XH = t->xHII;
     for (long r = 0; r < (long)N * N; r++)
       for (int z = 0; z < N; z++) {
         const long ir = r * N + z, ip = r * PR + z;
-        double SFR[MRX_TS_MAX_STEPS], fh[MRX_TS_MAX_STEPS], fi[MRX_TS_MAX_STEPS], fl[MRX_TS_MAX_STEPS];
+        double SFR[ORC_TS_MAX_STEPS], fh[ORC_TS_MAX_STEPS], fi[ORC_TS_MAX_STEPS], fl[ORC_TS_MAX_STEPS];
         double ans[2], der[5];
         ans[0] = x_e_box_prev[ip];
         ans[1] = Tk_box[ir];
-        for (int ct = 0; ct < s->nsteps; ct++) {
+        for (int ct = 0; ct < t->nsteps; ct++) {
           SFR[ct] = ((double)shells[(size_t)ct * nreal + ir] / pixel_volume) * unit_fac * len_fac / O_SOLAR_MASS;
           double xHII_call = x_e_box_prev[ip];
-          if (xHII_call > XH[MRX_X_INT_NXHII - 1] * 0.999)
-            xHII_call = XH[MRX_X_INT_NXHII - 1] * 0.999;
+          if (xHII_call > XH[ORC_XHII - 1] * 0.999)
+            xHII_call = XH[ORC_XHII - 1] * 0.999;
           else if (xHII_call < XH[0])
             xHII_call = 1.001 * XH[0];
-          const int lo = mrx_ts_locate_xHII_index(d, (float)xHII_call), hi = lo + 1;
-          fh[ct] = (s->freq_heat[hi][ct] - s->freq_heat[lo][ct]) / (XH[hi] - XH[lo]);
+          const int lo = o_locate_xHII(t, (float)xHII_call), hi = lo + 1;
+          fh[ct] = (t->freq_heat[hi][ct] - t->freq_heat[lo][ct]) / (XH[hi] - XH[lo]);
           fh[ct] *= (xHII_call - XH[lo]);
-          fh[ct] += s->freq_heat[lo][ct];
-          fi[ct] = (s->freq_ion[hi][ct] - s->freq_ion[lo][ct]) / (XH[hi] - XH[lo]);
+          fh[ct] += t->freq_heat[lo][ct];
+          fi[ct] = (t->freq_ion[hi][ct] - t->freq_ion[lo][ct]) / (XH[hi] - XH[lo]);
           fi[ct] *= (xHII_call - XH[lo]);
-          fi[ct] += s->freq_ion[lo][ct];
-          fl[ct] = (s->freq_lya[hi][ct] - s->freq_lya[lo][ct]) / (XH[hi] - XH[lo]);
+          fi[ct] += t->freq_ion[lo][ct];
+          fl[ct] = (t->freq_lya[hi][ct] - t->freq_lya[lo][ct]) / (XH[hi] - XH[lo]);
           fl[ct] *= (xHII_call - XH[lo]);
-          fl[ct] += s->freq_lya[lo][ct];
+          fl[ct] += t->freq_lya[lo][ct];
         }
-        o_evolve(d, s, zp, deltax[ip], SFR, fh, fi, fl, ans, der);
-        x_e_box_prev[ip] += der[0] * s->dzp;
+        o_evolve(p, t, zp, deltax[ip], SFR, fh, fi, fl, ans, der);
+        x_e_box_prev[ip] += der[0] * t->dzp;
         if (x_e_box_prev[ip] > 1)
           x_e_box_prev[ip] = (float)(1 - O_FRACT_FLOAT_ERR);
         else if (x_e_box_prev[ip] < 0)
           x_e_box_prev[ip] = 0;
-        if (Tk_box[ir] < O_MAX_TK) Tk_box[ir] += der[1] * s->dzp;
-        if (Tk_box[ir] < 0) Tk_box[ir] = (float)(O_TCMB * (1 + s->zp)); /* zp is double here */
+        if (Tk_box[ir] < O_MAX_TK) Tk_box[ir] += der[1] * t->dzp;
+        if (Tk_box[ir] < 0) Tk_box[ir] = (float)(O_TCMB * (1 + t->zp)); /* zp is double here */
         float xa;
-        TS_box[ir] = o_get_Ts(d, zp, deltax[ip], Tk_box[ir], x_e_box_prev[ip], (float)der[2], &xa);
+        TS_box[ir] = o_get_Ts(p, t, zp, deltax[ip], Tk_box[ir], x_e_box_prev[ip], (float)der[2], &xa);
         J_alpha_ave += der[2];
         xalpha_ave += xa;
         Xheat_ave += der[3];
         Xion_ave += der[4];
       }
-    if (shells_out) memcpy(shells_out, shells, sizeof(float) * (size_t)s->nsteps * nreal);
+    if (shells_out) memcpy(shells_out, shells, sizeof(float) * (size_t)t->nsteps * nreal);
     free(work);
     free(filt);
     free(spec);
@@ -368,211 +395,8 @@ int orc_compute_ts(const mrx_ts_params* p, const float* deltax, const float* sta
     out->volume_ave_xe = Ave_xe / total_n_cells;
     out->collapse_fraction = collapse_fraction;
     out->x_e_ave = x_e_ave;
-    out->no_light = s->homogeneous ? 0 : s->no_light;
-    out->homogeneous = s->homogeneous;
+    out->no_light = t->homogeneous ? 0 : t->no_light;
+    out->homogeneous = t->homogeneous;
   }
-  free(d);
-  free(s);
   return MRX_OK;
-}
-
-/* -------------------------------------------------- accessors for the tests */
-
-void* orc_ts_data_new(const mrx_ts_params* p)
-{
-  mrx_ts_data* d = (mrx_ts_data*)malloc(sizeof(mrx_ts_data));
-  if (d && mrx_ts_data_init(d, p) != MRX_OK) {
-    free(d);
-    return NULL;
-  }
-  return d;
-}
-
-void orc_ts_data_free(void* d) { free(d); }
-
-/* what = 0 T_RECFAST, 1 xion_RECFAST, 2 kappa_10, 3 kappa_10_elec, 4 kappa_10_pH,
- * 5 dtdz, 6 hubble, 7 dicke, 8 alpha_A, 9 drdz */
-double orc_ts_scalar(const void* dv, int what, double x)
-{
-  const mrx_ts_data* d = (const mrx_ts_data*)dv;
-  switch (what) {
-    case 0: return mrx_ts_T_RECFAST(d, (float)x);
-    case 1: return mrx_ts_xion_RECFAST(d, (float)x);
-    case 2: return mrx_ts_kappa_10(d, x);
-    case 3: return mrx_ts_kappa_10_elec(d, x);
-    case 4: return mrx_ts_kappa_10_pH(d, x);
-    case 5: return mrx_ts_dtdz(d, (float)x);
-    case 6: return mrx_ts_hubble(d, (float)x);
-    case 7: return mrx_ts_dicke(d, x);
-    case 8: return mrx_ts_alpha_A(x);
-    case 9: return mrx_ts_drdz(d, (float)x);
-    default: return 0;
-  }
-}
-
-double orc_ts_No(const void* dv, int which)
-{
-  const mrx_ts_data* d = (const mrx_ts_data*)dv;
-  return which == 0 ? d->No : (which == 1 ? d->N_b0 : (which == 2 ? d->f_H : d->f_He));
-}
-
-double orc_qag61(double (*f)(double, void*), double a, double b, double epsrel, double* abserr)
-{
-  return mrx_ts_qag61(f, NULL, a, b, epsrel, 1000, abserr);
-}
-
-double orc_brent(double (*f)(double, void*), double lo, double hi, double relerr)
-{
-  return mrx_ts_brent(f, NULL, lo, hi, relerr, 100);
-}
-
-double orc_spline(const double* x, const double* y, int n, double xv)
-{
-  static mrx_spline s;
-  if (mrx_spline_init(&s, x, y, n)) return NAN;
-  return mrx_spline_eval(&s, xv);
-}
-
-int orc_ts_plan_summary(const mrx_ts_params* p, double* R, double* zpp_edge, int* n_ops)
-{
-  mrx_ts_data* d = (mrx_ts_data*)orc_ts_data_new(p);
-  if (!d) return MRX_EINVAL;
-  mrx_ts_snap* s = (mrx_ts_snap*)malloc(sizeof(mrx_ts_snap));
-  int rc = mrx_ts_plan(s, d, p);
-  if (rc == MRX_OK) {
-    for (int i = 0; i < s->nsteps; i++) {
-      R[i] = s->R[i];
-      zpp_edge[i] = s->zpp_edge[i];
-      n_ops[i] = s->n_ops[i];
-    }
-    rc = s->homogeneous ? 1 : 0;
-  }
-  free(s);
-  free(d);
-  return rc;
-}
-
-/* Snapshot constants of the inhomogeneous spin-temperature step, flattened
- * for test references (float64 restatements of the per-cell update).  Needs
- * p->fixed_globals (the tables' two grid means come from p).  Layout:
- *   [0] nsteps [1] no_light [2] zp [3] dzp [4] dt_dzp [5] const_zp_prefactor
- *   [6] growth_factor_zp [7] dgrowth_factor_dzp [8] N_b0 [9] No [10] f_H
- *   [11] f_He [12] hubble((float)zp) [13] hubble_h [14] n_op
- *   [16 ...] R, dt_dzpp, dzpp, zpp, zpp_pow, sum_lyn (nsteps each),
- *   freq_heat, freq_ion, freq_lya ([14][nsteps] each), XH[14],
- *   3 kappa splines (HH, eH, pH): n, x[n], y[n], b[n], c[n], d[n],
- *   n_op ops: shell, kind, slot, weight.
- * Returns the number of doubles written, or a negative code. */
-int orc_ts_export(const mrx_ts_params* p, double* out, int cap)
-{
-  if (!p->fixed_globals) return MRX_EINVAL;
-  mrx_ts_data* d = (mrx_ts_data*)orc_ts_data_new(p);
-  if (!d) return MRX_EINVAL;
-  mrx_ts_snap* s = (mrx_ts_snap*)calloc(1, sizeof(mrx_ts_snap));
-  int rc = mrx_ts_plan(s, d, p);
-  if (rc == MRX_OK && s->homogeneous) rc = MRX_EUNSUPPORTED;
-  if (rc == MRX_OK) {
-    s->collapse_fraction = p->fixed_collapse_fraction;
-    s->x_e_ave = p->fixed_x_e_ave;
-    rc = mrx_ts_build_tables(s, d, p);
-  }
-  int n = 0;
-  const int ns = s->nsteps;
-  int nop = 0;
-  for (int c = 0; c < ns; c++) nop += s->n_ops[c];
-  const int need = 16 + 6 * ns + 3 * MRX_X_INT_NXHII * ns + MRX_X_INT_NXHII + 3 * (1 + 5 * 27) + 4 * nop;
-  if (rc == MRX_OK && cap < need) rc = MRX_EINVAL;
-  if (rc == MRX_OK) {
-    double* o = out;
-    o[0] = ns;
-    o[1] = s->no_light;
-    o[2] = s->zp;
-    o[3] = s->dzp;
-    o[4] = s->dt_dzp;
-    o[5] = s->const_zp_prefactor;
-    o[6] = s->growth_factor_zp;
-    o[7] = s->dgrowth_factor_dzp;
-    o[8] = d->N_b0;
-    o[9] = d->No;
-    o[10] = d->f_H;
-    o[11] = d->f_He;
-    o[12] = mrx_ts_hubble(d, (float)s->zp);
-    o[13] = p->hubble_h;
-    o[14] = nop;
-    o[15] = 0;
-    n = 16;
-    for (int c = 0; c < ns; c++) out[n++] = s->R[c];
-    for (int c = 0; c < ns; c++) out[n++] = s->dt_dzpp[c];
-    for (int c = 0; c < ns; c++) out[n++] = s->dzpp[c];
-    for (int c = 0; c < ns; c++) out[n++] = s->zpp[c];
-    for (int c = 0; c < ns; c++) out[n++] = s->zpp_pow[c];
-    for (int c = 0; c < ns; c++) out[n++] = s->sum_lyn[c];
-    for (int x = 0; x < MRX_X_INT_NXHII; x++)
-      for (int c = 0; c < ns; c++) out[n++] = s->freq_heat[x][c];
-    for (int x = 0; x < MRX_X_INT_NXHII; x++)
-      for (int c = 0; c < ns; c++) out[n++] = s->freq_ion[x][c];
-    for (int x = 0; x < MRX_X_INT_NXHII; x++)
-      for (int c = 0; c < ns; c++) out[n++] = s->freq_lya[x][c];
-    for (int x = 0; x < MRX_X_INT_NXHII; x++) out[n++] = d->xi_XHII[x];
-    const mrx_spline* sp[3] = {&d->kappa_HH, &d->kappa_eH, &d->kappa_pH};
-    for (int k = 0; k < 3; k++) {
-      const int m = sp[k]->n < 27 ? sp[k]->n : 27;
-      out[n++] = m;
-      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->x[i] : 0.0;
-      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->y[i] : 0.0;
-      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->b[i] : 0.0;
-      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->c[i] : 0.0;
-      for (int i = 0; i < 27; i++) out[n++] = i < m ? sp[k]->d[i] : 0.0;
-    }
-    for (int c = 0; c < ns; c++)
-      for (int k = 0; k < s->n_ops[c]; k++) {
-        out[n++] = c;
-        out[n++] = s->ops[c][k].kind;
-        out[n++] = s->ops[c][k].slot;
-        out[n++] = s->ops[c][k].weight;
-      }
-  }
-  free(s);
-  free(d);
-  return rc == MRX_OK ? n : rc;
-}
-
-/* Builds the snapshot's tables (threaded, each shell's integrals sharing node
- * values) and recomputes every frequency integral alone with
- * mrx_ts_integrate_over_nu (no shared values, one thread), with the shell's
- * lower limit formed as ComputeTs.c does; returns how many of the
- * 3 x 14 x nsteps entries differ in any bit, or a negative error. */
-int orc_ts_tables_recheck(const mrx_ts_params* p, double collapse_fraction, double x_e_ave)
-{
-  const double nu_over_ev = 1.60217646e-12 / 6.6262e-27; /* eV -> Hz, the tables' constant */
-  mrx_ts_data* d = (mrx_ts_data*)orc_ts_data_new(p);
-  if (!d) return MRX_EINVAL;
-  mrx_ts_snap* s = (mrx_ts_snap*)calloc(1, sizeof(mrx_ts_snap));
-  int rc = s ? mrx_ts_plan(s, d, p) : MRX_ENOMEM;
-  if (rc == MRX_OK && s->homogeneous) rc = MRX_EUNSUPPORTED;
-  if (rc == MRX_OK) {
-    s->collapse_fraction = collapse_fraction;
-    s->x_e_ave = x_e_ave;
-    rc = mrx_ts_build_tables(s, d, p);
-  }
-  int bad = 0;
-  if (rc == MRX_OK) {
-    for (int ct = 0; ct < s->nsteps; ct++) {
-      const double zp = s->zp, prev = (ct == 0) ? zp : s->zpp_edge[ct - 1];
-      const double zpp = (s->zpp_edge[ct] + prev) * 0.5;
-      const double lower = fmax(mrx_ts_nu_tau_one(d, p, zp, zpp, x_e_ave, x_e_ave),
-                                p->nu_xray_gal_threshold * nu_over_ev);
-      for (int x = 0; x < MRX_X_INT_NXHII; x++) {
-        const double h = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 0);
-        const double i = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 1);
-        const double l = mrx_ts_integrate_over_nu(d, p, zp, d->xi_XHII[x], lower, 2);
-        bad += memcmp(&h, &s->freq_heat[x][ct], sizeof h) != 0;
-        bad += memcmp(&i, &s->freq_ion[x][ct], sizeof i) != 0;
-        bad += memcmp(&l, &s->freq_lya[x][ct], sizeof l) != 0;
-      }
-    }
-  }
-  free(s);
-  free(d);
-  return rc == MRX_OK ? bad : rc;
 }

@@ -61,6 +61,7 @@ def test_struct_mirrors_match_library():
     assert L.mrx_abi_sizeof(5) == C.sizeof(abi.HiiBatchBox)
     assert L.mrx_abi_sizeof(6) == C.sizeof(abi.GridParams)
     assert L.mrx_abi_sizeof(7) == C.sizeof(abi.GridStats)
+    assert L.mrx_abi_sizeof(9) == C.sizeof(abi.TsTables)
     assert L.mrx_abi_sizeof(8) == C.sizeof(abi.TbVelParams)
 
 

@@ -40,7 +40,9 @@ def test_oracle_reproduces_ts_fixture():
     out = ts_case()
     g = load("ts_n32.npz")
     for k, v in out.items():
-        assert np.array_equal(v, g[k]), k
+        # the host tables use numpy's vectorised transcendental functions, whose last
+        # bit can depend on the CPU's instruction set: a 1e-6 bound, far inside parity
+        np.testing.assert_allclose(v, g[k], rtol=1e-6, atol=0, err_msg=k)
 
 
 @pytest.mark.gpu

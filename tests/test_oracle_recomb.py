@@ -1,15 +1,17 @@
 """Oracle: inhomogeneous recombinations (recombinations.c, find_HII_bubbles.c:155-477).
 
-The host tables (recomb_tables.c, shared by the library and the oracle) are
-pinned against independent evaluations: the MHR00 normalisation A(z) against
-scipy's quad in ln D; the equilibrium neutral fraction against its quadratic
-and its small-chi limit; the self-shielding fit at Delta -> 0; the rate
-integral against scipy's quad of the same integrand; the splined table
-against the integral at its knots and the reference's index rounding and
-clamps; dt/dz against -1/((1+z) H(z)).  The bubble finder's recombination
-terms are pinned by closed forms: N_rec = 0 leaves xH unchanged, a uniform
-N_rec shifts the threshold to n_f (1 + N_rec), and the N_rec update of a
-uniform box is RR(z, Gamma12) |dt/dz| zstep (1 - xH).
+The oracle's restatement (oracle/recomb.c: its own natural spline and QAG-61,
+sharing no code with the library) is pinned against independent evaluations:
+the MHR00 normalisation A(z) against scipy's quad in ln D; the equilibrium
+neutral fraction against its quadratic and its small-chi limit; the
+self-shielding fit at Delta -> 0; the rate integral against scipy's quad of the
+same integrand; the splined table against the integral at its knots and the
+reference's index rounding and clamps; dt/dz against -1/((1+z) H(z)).  The
+library's tables (recomb_tables.c, reached through libmrx_hip.so's host
+functions) equal the oracle's.  The bubble finder's recombination terms are
+pinned by closed forms: N_rec = 0 leaves xH unchanged, a uniform N_rec shifts
+the threshold to n_f (1 + N_rec), and the N_rec update of a uniform box is
+RR(z, Gamma12) |dt/dz| zstep (1 - xH).
 """
 import ctypes as C
 import math
@@ -18,15 +20,59 @@ import numpy as np
 import pytest
 from scipy import integrate, interpolate
 
+from meraxes_amd import _lib
 from meraxes_amd import params as P
 from oracle import oracle as O
 
 Y_HE = 0.24
 
 
+class _Rr:
+    """The oracle's recombination functions under the names the tests use."""
+
+    def __init__(self, L):
+        self.L = L
+
+    def A(self, z):
+        return self.L.orc_rr_A(z)
+
+    def C(self, z):
+        return self.L.orc_rr_C(z)
+
+    def beta(self, z):
+        return self.L.orc_rr_beta(z)
+
+    def rate(self, z, g):
+        return self.L.orc_rr_rate(float(np.float32(z)), g)
+
+    def recombination_rate(self, z, g, T4):
+        return self.L.orc_rr_recombination_rate(z, g, T4)
+
+
 @pytest.fixture(scope="module")
 def rr():
-    L = O.rr_tables_lib()
+    L = O.rr_lib()
+    assert L.orc_rr_build(P.HUBBLE_H, P.OMEGA_M, P.BARYON_FRAC, Y_HE) == 0
+    rhoc = 3.0 * (3.2407789e-18 * P.HUBBLE_H) ** 2 / (8.0 * math.pi * 6.672e-8)
+    No = rhoc * P.BARYON_FRAC * P.OMEGA_M * (1 - Y_HE) / 1.6726e-24
+    return L, _Rr(L), No
+
+
+@pytest.fixture(scope="module")
+def lib_rr():
+    """The library's tables (recomb_tables.c) for the same cosmology."""
+    L = _lib.load()
+    d = C.c_double
+    L.mrx_rr_No.argtypes = [d, d, d, d]
+    L.mrx_rr_No.restype = d
+    L.mrx_rr_init.argtypes = [C.c_void_p, d, d]
+    L.mrx_rr_init.restype = C.c_int
+    for name, args in (("mrx_rr_A_MHR", [C.c_void_p, d]), ("mrx_rr_C_MHR", [C.c_void_p, d]),
+                       ("mrx_rr_beta_MHR", [C.c_void_p, d]), ("mrx_rr_rate", [C.c_void_p, C.c_float, d]),
+                       ("mrx_rr_recombination_rate", [C.c_void_p, d, d, d, C.c_int])):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = d
     No = L.mrx_rr_No(P.HUBBLE_H, P.OMEGA_M, P.BARYON_FRAC, Y_HE)
     buf = C.create_string_buffer(2_000_000)
     assert L.mrx_rr_init(buf, No, Y_HE) == 0
@@ -47,14 +93,14 @@ def B_fn(z):
 def test_mhr_parameters_and_normalisation(rr):
     L, t, _ = rr
     for z in (2.0, 4.5, 8.0, 12.3, 20.0, 45.0):
-        assert L.mrx_rr_C_MHR(t, z) == pytest.approx(C_fn(z), abs=1e-12)
-        assert L.mrx_rr_beta_MHR(t, z) == pytest.approx(B_fn(z), abs=1e-12)
+        assert t.C(z) == pytest.approx(C_fn(z), abs=1e-12)
+        assert t.beta(z) == pytest.approx(B_fn(z), abs=1e-12)
         s = 2 * 7.61 / (3 * (1 + z))
         f = (lambda u: math.exp(-(math.exp(-2 / 3 * u) - C_fn(z)) ** 2 / (2 * s * s)) * math.exp((B_fn(z) + 1) * u))
         norm, _ = integrate.quad(f, math.log(1e-25), math.log(1e25), points=[0.0], limit=500, epsrel=1e-11)
         # A is splined between integer z; at integers it is 1 / the integral (QAG to 1e-3)
         if z == int(z):
-            assert L.mrx_rr_A_MHR(t, z) * norm == pytest.approx(1.0, rel=1e-4)
+            assert t.A(z) * norm == pytest.approx(1.0, rel=1e-4)
 
 
 def test_neutral_fraction_and_self_shielding(rr):
@@ -63,7 +109,7 @@ def test_neutral_fraction_and_self_shielding(rr):
     aB = 2.59e-13
     for n in (1e-6, 1e-4, 1e-2, 1.0):
         for g in (1e-3, 0.1, 1.0, 30.0):
-            chi = L.mrx_rr_neutral_fraction(Y_HE, n, 1.0, g, 1)
+            chi = L.orc_rr_neutral_fraction(Y_HE, n, 1.0, g)
             a = (1 + corr) * n * aB / (g * 1e-12)
             if a < 1e-5:
                 assert chi == pytest.approx(a, rel=1e-12)  # the small-chi branch returns chi itself
@@ -75,44 +121,72 @@ def test_neutral_fraction_and_self_shielding(rr):
                 assert chi == pytest.approx(stable, rel=max(1e-12, 4e-16 * b * b))
                 assert 0 < chi < 1
     # Rahmati et al. fit: no attenuation at vanishing density, strong at high density
-    assert L.mrx_rr_Gamma_SS(0.7, 1e-12, 1.0, 8.0) == pytest.approx(0.7, rel=1e-9)
-    assert L.mrx_rr_Gamma_SS(0.7, 1e4, 1.0, 8.0) < 0.05 * 0.7
+    assert L.orc_rr_Gamma_SS(0.7, 1e-12, 1.0, 8.0) == pytest.approx(0.7, rel=1e-9)
+    assert L.orc_rr_Gamma_SS(0.7, 1e4, 1.0, 8.0) < 0.05 * 0.7
 
 
 def test_rate_integral_and_table(rr):
     L, t, No = rr
     for z, g in ((6.0, 0.3), (8.0, 1.0), (12.4, 5.0)):
-        A, Cz, Bz = L.mrx_rr_A_MHR(t, z), L.mrx_rr_C_MHR(t, z), L.mrx_rr_beta_MHR(t, z)
+        A, Cz, Bz = t.A(z), t.C(z), t.beta(z)
         nbar = No * (1 + z) ** 3
 
         def f(u):
             D = math.exp(u)
-            gam = L.mrx_rr_Gamma_SS(g, D, 1.0, z)
-            xe = 1.0 - L.mrx_rr_neutral_fraction(Y_HE, nbar * D, 1.0, gam, 1)
+            gam = L.orc_rr_Gamma_SS(g, D, 1.0, z)
+            xe = 1.0 - L.orc_rr_neutral_fraction(Y_HE, nbar * D, 1.0, gam)
             Pd = A * math.exp(-0.5 * ((D ** (-2 / 3) - Cz) / (2 * 7.61 / (3 * (1 + z)))) ** 2) * D ** Bz
             return nbar * D * Pd * 2.59e-13 * xe * xe * D * D
 
         ref, _ = integrate.quad(f, math.log(0.01), math.log(200), limit=200, epsrel=1e-10)
-        assert L.mrx_rr_recombination_rate(t, z, g, 1.0, 1) == pytest.approx(ref, rel=1e-2)
+        assert t.recombination_rate(z, g, 1.0) == pytest.approx(ref, rel=1e-2)
     # the spline passes through its knots: row z_ct = 40 is z = 8, knot k is ln Gamma = -10 + k * 0.1f
     for k in (0, 37, 100, 149):
         lnG = -10.0 + float(np.float32(k) * np.float32(0.1))
         g = float(np.float32(math.exp(lnG)))
-        direct = L.mrx_rr_recombination_rate(t, float(np.float32(40 * np.float32(0.2))), g, 1.0, 1)
-        assert L.mrx_rr_rate(t, 8.0, math.exp(lnG)) == pytest.approx(direct, rel=1e-12)
+        direct = t.recombination_rate(float(np.float32(40 * np.float32(0.2))), g, 1.0)
+        assert t.rate(8.0, math.exp(lnG)) == pytest.approx(direct, rel=1e-12)
     # nearest redshift row (z / 0.2 rounded), and the clamps
-    assert L.mrx_rr_rate(t, 8.09, 1.0) == L.mrx_rr_rate(t, 8.0, 1.0)
-    assert L.mrx_rr_rate(t, 8.11, 1.0) == L.mrx_rr_rate(t, 8.2, 1.0)
-    assert L.mrx_rr_rate(t, 8.0, math.exp(-10.5)) == 0.0
-    assert L.mrx_rr_rate(t, -3.0, 1.0) == L.mrx_rr_rate(t, 0.0, 1.0)
+    assert t.rate(8.09, 1.0) == t.rate(8.0, 1.0)
+    assert t.rate(8.11, 1.0) == t.rate(8.2, 1.0)
+    assert t.rate(8.0, math.exp(-10.5)) == 0.0
+    assert t.rate(-3.0, 1.0) == t.rate(0.0, 1.0)
+    # above the last knot (ln Gamma12 > 4.9) the reference's GSL spline has no value
+    assert math.isnan(t.rate(8.0, math.exp(4.95)))
 
 
 def test_dtdz_closed_form():
-    L = O.rr_tables_lib()
+    L = O.rr_lib()
     for z in (0.0, 6.0, 8.4, 20.0):
         z = float(np.float32(z))  # dtdz takes a float redshift
         H = 3.2407789e-18 * P.HUBBLE_H * math.sqrt(P.OMEGA_M * (1 + z) ** 3 + P.OMEGA_L)
-        assert L.mrx_cosmo_dtdz(P.OMEGA_L, P.OMEGA_M, P.HUBBLE_H, z) == pytest.approx(-1 / ((1 + z) * H), rel=1e-9)
+        assert L.orc_dtdz(P.OMEGA_L, P.OMEGA_M, P.HUBBLE_H, z) == pytest.approx(-1 / ((1 + z) * H), rel=1e-9)
+
+
+def test_library_tables_equal_the_oracles(rr, lib_rr):
+    """The library's table build (threaded, recomb_tables.c) against the oracle's
+    restatement: the MHR parameters, every 7th knot of every 10th redshift row,
+    and the splined rate between knots (the same algorithms, written apart)."""
+    L, t, No = rr
+    Ll, buf, No_l = lib_rr
+    assert No_l == pytest.approx(No, rel=1e-14)
+    # (A's knots are z = 2..61: above 61 the reference's spline has no value, and
+    # the rate table's rows stop at z = 59.8)
+    for z in (2.0, 3.3, 5.5, 9.0, 30.0, 60.5):
+        assert Ll.mrx_rr_A_MHR(buf, z) == pytest.approx(t.A(z), rel=1e-12)
+        assert Ll.mrx_rr_C_MHR(buf, z) == pytest.approx(t.C(z), rel=1e-14)
+        assert Ll.mrx_rr_beta_MHR(buf, z) == pytest.approx(t.beta(z), rel=1e-14)
+    for zc in range(0, 300, 10):
+        zf = float(np.float32(zc * np.float32(0.2)))
+        for g in range(0, 150, 7):
+            want = L.orc_rr_table(zc, g)
+            gam = float(np.float32(math.exp(L.orc_rr_lnG(g))))
+            assert Ll.mrx_rr_recombination_rate(buf, zf, gam, 1.0, 1) == pytest.approx(want, rel=1e-12)
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        z = float(np.float32(rng.uniform(0, 59)))
+        g = math.exp(rng.uniform(-9.9, 4.8))
+        assert Ll.mrx_rr_rate(buf, z, g) == pytest.approx(t.rate(z, g), rel=1e-11)
 
 
 def uniform_box(N, s_val, d_val=0.0):
@@ -146,8 +220,8 @@ def test_bubble_finder_recombination_terms(rr):
     assert (r1["r_bubble"] == 0).all() and (r1["xH"] == 0).all() and (r1["Gamma12"] == 0).all()
     # N_rec update of the fully ionised uniform box: z_eff = z, dN = RR(z, Gamma12) |dt/dz| zstep
     g12 = float(r0["Gamma12"][0, 0, 0])
-    dtdz = abs(L.mrx_cosmo_dtdz(p.omega_lambda, p.omega_m, p.hubble_h, z) / p.hubble_h)
-    dN = L.mrx_rr_rate(t, z, g12) * float(np.float32(dtdz)) * float(np.float32(zprev - z))
+    dtdz = abs(L.orc_dtdz(p.omega_lambda, p.omega_m, p.hubble_h, z) / p.hubble_h)
+    dN = t.rate(z, g12) * float(np.float32(dtdz)) * float(np.float32(zprev - z))
     assert dN > 0
     np.testing.assert_array_equal(r0["N_rec"][:, :, :N], np.float32(dN))
     assert (r0["N_rec"][:, :, N:] == 0).all()
