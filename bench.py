@@ -61,9 +61,11 @@ def kernel_bytes(name, N, F, radii=1.0):
     """
     if name.startswith("ts_"):
         name, F = name[3:], 1
-        if name in ("ix_filter", "colpass_y_inv", "c2r_z"):
+        if name in ("ix_filter", "colpass_y_inv", "c2r_z", "colpass_y_fwd", "colpass_x_fwd"):
             # sharp-k shells: only the tiles holding the ball of kept modes are
-            # read (and, in the x/y passes, written), so the bytes depend on R
+            # read (and, in the x/y passes, written), and the forward passes of
+            # the working grid transform only the tiles the shells reusing it
+            # read, so the bytes depend on R
             return None
         if name == "cell":
             name = "ts_cell"
@@ -87,65 +89,114 @@ def kernel_bytes(name, N, F, radii=1.0):
     return table.get(name)
 
 
-def cpu_baseline(snapshot):
-    """The oracle (ref32 restatement of the reference's CPU path) on a bounded sample.
+def cpu_info():
+    """The host's CPU model, its logical CPUs and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "logical_cpus": os.cpu_count()}
 
-    bubbles: find_HII_bubbles over the full ladder of a 256^3, 200 Mpc/h box
-    (the 512^3 / 400 Mpc/h cell size).  snapshot: spin temperature + bubble
-    finder with x_e + brightness temperature on a 160^3, 125 Mpc/h box (same
-    cell size), the same z and history as the GPU step.
-    """
+
+def _cpu_snapshot(n, box, threads):
+    """One snapshot of the oracle (Ts + bubbles with x_e + dTb) on an n^3 box; seconds by part."""
     import numpy as np
 
+    from meraxes_amd import params as P
+    from meraxes_amd.snapshots import SyntheticRun
+    from oracle import oracle as O
+
+    O.set_threads(threads)
+    zz = P.snapshot_redshifts()
+    snap = 24
+    z = float(zz[snap])
+    d, s, w = SyntheticRun(n, box, seed=1).inputs(snap)
+    nh = 8
+    hist = np.stack([w * np.float32(((1 + z) / (1 + zz[max(snap - k, 0)])) ** 3) for k in range(nh)])
+    fcoll = np.zeros(len(zz))
+    fcoll[:snap] = np.linspace(1e-4, 5e-3, snap)
+    tsp = P.ts_params(n, box, snap, zz, fcoll, nh, efficiency=20.0)
+    xe0 = np.full_like(d, 2e-3)
+    Tk0 = np.full((n, n, n), 50.0, np.float32)
+    p = P.hii_params(n, box, z, filter_type=0, efficiency=20.0, uvb_flag=1, include_spin_temp=1)
+    tp = P.tb_params(n, z, include_spin_temp=1)
+    t0 = time.perf_counter()
+    ts = O.compute_ts(tsp, d, s, w, hist, xe0, Tk0)
+    t1 = time.perf_counter()
+    r = O.find_hii_bubbles(p, d, s, w, x_e=ts["x_e_box"])
+    O.brightness_temp(tp, r["xH"], d, ts["TS_box"])
+    t2 = time.perf_counter()
+    return r["n_radii"], t2 - t0, t1 - t0, O.get_threads()
+
+
+def cpu_baseline(snapshot):
+    """The oracle (ref32 restatement of the reference's CPU path) on bounded samples.
+
+    snapshot: spin temperature + bubble finder with x_e + brightness
+    temperature of z = 8.41 on boxes with the 512^3 / 400 Mpc/h cell size:
+    160^3 (125 Mpc/h) on every host core the process may use, and 96^3
+    (75 Mpc/h) on one thread (the reference runs one thread per MPI rank).
+    bubbles: find_HII_bubbles over the full ladder of a 256^3, 200 Mpc/h box.
+    """
     from meraxes_amd import params as P
     from meraxes_amd import synth
     from oracle import oracle as O
 
+    info = cpu_info()
     if not snapshot:
         n, box = 256, 200.0
         d, s, w = synth.snapshot_inputs(n, 1)
         p = P.hii_params(n, box, 8.0, filter_type=0, efficiency=20.0, uvb_flag=1)
+        O.set_threads(info["nproc"])
         t0 = time.perf_counter()
         r = O.find_hii_bubbles(p, d, s, w)
         dt = time.perf_counter() - t0
         S = r["n_radii"]
-        what = f"find_HII_bubbles, {n}^3 box {box:g} Mpc/h z=8, full ladder S={S}, F=3, UVB on, top-hat"
-    else:
-        n, box = 160, 125.0
-        zz = P.snapshot_redshifts()
-        snap = 24
-        z = float(zz[snap])
-        d, s, w = synth.snapshot_inputs(n, 1)
-        nh = 8
-        hist = np.stack([w * np.float32(((1 + z) / (1 + zz[max(snap - k, 0)])) ** 3) for k in range(nh)])
-        fcoll = np.zeros(len(zz))
-        fcoll[:snap] = np.linspace(1e-4, 5e-3, snap)
-        tsp = P.ts_params(n, box, snap, zz, fcoll, nh, efficiency=20.0)
-        xe0 = np.full_like(d, 2e-3)
-        Tk0 = np.full((n, n, n), 50.0, np.float32)
-        p = P.hii_params(n, box, z, filter_type=0, efficiency=20.0, uvb_flag=1, include_spin_temp=1)
-        tp = P.tb_params(n, z, include_spin_temp=1)
-        t0 = time.perf_counter()
-        ts = O.compute_ts(tsp, d, s, w, hist, xe0, Tk0)
-        r = O.find_hii_bubbles(p, d, s, w, x_e=ts["x_e_box"])
-        O.brightness_temp(tp, r["xH"], d, ts["TS_box"])
-        dt = time.perf_counter() - t0
-        S = r["n_radii"]
-        what = (f"ComputeTs (40 shells) + find_HII_bubbles (F=4, S={S}) + dTb, {n}^3 box {box:g} Mpc/h "
-                f"z={z:.3f}")
+        return dict(value=n**3 * S / dt, unit="cells*radii/s", cores=O.get_threads(), kind="port",
+                    sample=f"oracle ref32: find_HII_bubbles, {n}^3 box {box:g} Mpc/h z=8, S={S}, F=3; {dt:.1f} s",
+                    **info)
+    n, box = 160, 125.0
+    S, dt, dts, threads = _cpu_snapshot(n, box, info["nproc"])
+    n1, box1 = 96, 75.0
+    S1, dt1, dts1, _ = _cpu_snapshot(n1, box1, 1)
     return {
         "value": n**3 * S / dt,
         "unit": "cells*radii/s",
-        "cores": O.get_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": f"oracle ref32: {what}; {dt:.1f} s wall",
+        "sample": (f"oracle ref32, ComputeTs (40 shells) + find_HII_bubbles (F=4, S={S}) + dTb at z=8.414, "
+                   f"{n}^3 box {box:g} Mpc/h, {threads} threads: {dt:.1f} s (Ts {dts:.1f} s, of which the "
+                   f"single-threaded numpy host tables about 2.3 s)"),
+        "single_thread": {"value": n1**3 * S1 / dt1, "unit": "cells*radii/s", "cores": 1,
+                          "sample": f"the same step on a {n1}^3, {box1:g} Mpc/h box (S={S1}), one thread: "
+                                    f"{dt1:.1f} s (Ts {dts1:.1f} s)"},
+        **info,
     }
 
 
-def copy_ceiling(torch, nbytes=1 << 30, reps=10):
-    """Practical HBM ceiling: a 1 GiB device-to-device copy, bytes read + written per second."""
+def copy_ceiling(torch, ctx, nbytes=1 << 30, reps=10):
+    """Practical HBM ceilings for a 1 GiB device-to-device copy (bytes read + written
+    per second): the library's streaming copy kernel (16-byte loads, four in flight
+    per thread, non-temporal stores; HIP events on its stream) and torch's copy_."""
+    import ctypes as C
+
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    ms = C.c_double()
+    ctx.check(ctx.lib.mrx_stream_copy(ctx.ptr, C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()), nbytes, reps,
+                                      C.byref(ms)), "stream_copy")
+    assert torch.equal(dst[:: 1 << 16], src[:: 1 << 16])
+    t_stream = ms.value / 1e3
     dst.copy_(src)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -156,8 +207,9 @@ def copy_ceiling(torch, nbytes=1 << 30, reps=10):
     torch.cuda.synchronize()
     t = e0.elapsed_time(e1) / 1e3 / reps
     del src, dst
-    return {"GB_s": 2 * nbytes / t / 1e9, "frac_of_peak": 2 * nbytes / t / HBM_PEAK, "bytes": nbytes,
-            "kernel": "torch copy_ (1 GiB, 10 repetitions, CUDA events on the current stream)"}
+    return {"GB_s": 2 * nbytes / t_stream / 1e9, "frac_of_peak": 2 * nbytes / t_stream / HBM_PEAK, "bytes": nbytes,
+            "kernel": "mrx_stream_copy (1 GiB, 10 repetitions)",
+            "torch_copy_GB_s": 2 * nbytes / t / 1e9, "torch_copy_frac_of_peak": 2 * nbytes / t / HBM_PEAK}
 
 
 def measured_traffic(kernel, N):
@@ -558,6 +610,7 @@ def main():
     from meraxes_amd import params as P
     from meraxes_amd import reion, synth
     from meraxes_amd._lib import Context
+    from meraxes_amd.snapshots import SyntheticRun
 
     if args.workload == "batched":
         ctx = Context(0)
@@ -600,7 +653,9 @@ def main():
     snap = 24  # z = 8.41 in the 30-snapshot run from z = 20 to 6
     z = float(zz[snap])
     log(f"[bench] generating {N}^3 inputs ({args.workload})")
-    d, s, w = synth.snapshot_inputs(N, 0)
+    # the full-snapshot configuration's density: one seeded k^-2 field grown
+    # linearly as 1/(1+z) from z = 20, sources from the grown field
+    d, s, w = SyntheticRun(N, L, seed=0).inputs(snap)
     dev = torch.device("cuda:0")
     dd, ds, dw = (torch.from_numpy(a).to(dev) for a in (d, s, w))
     del d, s, w
@@ -726,8 +781,18 @@ def main():
         "roofline": roof,
         "kernels": per_kernel,
     }
-    out["hbm_copy_ceiling"] = copy_ceiling(torch)
-    log(f"[bench] device copy ceiling {out['hbm_copy_ceiling']['GB_s']:.0f} GB/s (read + write)")
+    # the path at the kernels' own algorithmic bytes: every launch whose bytes are
+    # known (the sharp-k shells' pruned passes are not counted) over the step time
+    known = sum((kernel_bytes(k, N, F, radii) or 0) * n for k, (ms, n) in prof.items()) / args.steps
+    known_ms = sum(ms for k, (ms, n) in prof.items() if kernel_bytes(k, N, F, radii)) / args.steps
+    out["path_kernel_bytes"] = {
+        "bytes_per_step": known, "frac_of_peak": known / dt / HBM_PEAK,
+        "device_ms_of_those_kernels": known_ms, "frac_of_peak_over_their_device_time": known / (known_ms / 1e3) / HBM_PEAK,
+        "note": "sum over launches of each kernel's algorithmic bytes (kernels[].GB_s), over the step's wall time "
+                "and over those kernels' own device time; the pruned sharp-k shell passes are left out"}
+    out["hbm_copy_ceiling"] = copy_ceiling(torch, ctx)
+    log(f"[bench] device copy ceiling {out['hbm_copy_ceiling']['GB_s']:.0f} GB/s streaming kernel, "
+        f"{out['hbm_copy_ceiling']['torch_copy_GB_s']:.0f} GB/s torch copy_ (read + write)")
     if snapshot and not args.no_extras:
         del dd, ds, dw, hist, xH, rb, J21, dT, zin, J21i, xe0, Tk0, xe_prev, Tk, xe_box, TS
         torch.cuda.empty_cache()

@@ -151,6 +151,9 @@ typedef struct mrx_ts_globals {
  * find_HII_bubbles_gpu.cu:151-193).
  */
 int mrx_init(int device, mrx_ctx** ctx);
+/* Number of visible devices (0 when none or on error): init_CUDA picks
+ * rank % count (meraxes_gpu.cu:142). */
+int mrx_device_count(void);
 int mrx_free(mrx_ctx* ctx);
 const char* mrx_last_error(const mrx_ctx* ctx);
 /* Synchronise the context's stream (all entry points are synchronous on return
@@ -501,6 +504,16 @@ int mrx_construct_baryon_grids(mrx_ctx* ctx, const mrx_grid_params* p, long n_ga
                                const int* type, const double* fesc_weighted_gsm, const double* gross_stellar_mass,
                                const double* black_hole_mass, const double* effective_bhm, float* stars,
                                float* weighted_sfr, float* sfr, float* sfr_histories, mrx_grid_stats* stats);
+
+/* ---------------------------------------------------------------------------
+ * Device copy rate.  No counterpart in the reference: copies `bytes` (a
+ * multiple of 16, 16-byte aligned device buffers) from src to dst reps times
+ * on the context stream with a streaming kernel (16-byte loads, several in
+ * flight per thread, non-temporal stores) after one warm-up copy, and writes
+ * the average milliseconds per copy measured with HIP events on that stream.
+ * For callers (the benchmark) that set achieved bandwidth beside the device's
+ * practical copy rate. */
+int mrx_stream_copy(mrx_ctx* ctx, void* dst, const void* src, size_t bytes, int reps, double* ms_per_copy);
 
 /* sizeof of the ABI structs (0 mrx_hii_params, 1 mrx_hii_globals, 2 mrx_tb_params,
  * 3 mrx_ts_params, 4 mrx_ts_globals, 5 mrx_hii_batch_box, 6 mrx_grid_params,

@@ -21,7 +21,7 @@ EXPORTS = (
     "mrx_hii_batch_create", "mrx_hii_batch_run", "mrx_hii_batch_describe", "mrx_hii_batch_free",
     "mrx_power_spectrum", "mrx_power_spectrum_adjoint", "mrx_snapshot_adjoint", "mrx_construct_baryon_grids",
     "mrx_brightness_temp_vel", "mrx_find_hii_bubbles_rec", "mrx_brightness_temp_noalloc", "mrx_ts_snapshot_tables",
-    "mrx_ts_last_shells",
+    "mrx_ts_last_shells", "mrx_stream_copy", "mrx_device_count",
 )
 
 # every symbol include/mrx_io.h declares (libmrx_io.so)
@@ -48,6 +48,7 @@ def load():
     vp, i, d, f = C.c_void_p, C.c_int, C.c_double, C.c_float
     ctxp = C.c_void_p
     L.mrx_init.argtypes = [i, C.POINTER(ctxp)]
+    L.mrx_device_count.argtypes = []
     L.mrx_free.argtypes = [ctxp]
     L.mrx_last_error.argtypes = [ctxp]
     L.mrx_last_error.restype = C.c_char_p
@@ -64,6 +65,7 @@ def load():
     L.mrx_compute_ts.argtypes = [ctxp, C.POINTER(TsParams)] + [vp] * 8 + [C.POINTER(TsGlobals)]
     L.mrx_ts_snapshot_tables.argtypes = [C.POINTER(TsParams), d, d, C.POINTER(TsTables)]
     L.mrx_ts_last_shells.argtypes = [ctxp, i, i, vp, i]
+    L.mrx_stream_copy.argtypes = [ctxp, vp, vp, C.c_size_t, i, C.POINTER(C.c_double)]
     L.mrx_profile.argtypes = [ctxp, i]
     L.mrx_profile_read.argtypes = [ctxp, i, C.c_char_p, i, vp, vp]
     L.mrx_hii_batch_create.argtypes = [ctxp, C.POINTER(HiiBatchBox), i, i, C.POINTER(vp)]

@@ -52,7 +52,9 @@ def build(verbose=False):
     # the HIP translation unit and the C object are linked together through
     # the linker so hipcc does not treat the object as a HIP source
     grid = os.path.join(CSRC, "grid.hip")  # galaxy gridding (rocPRIM sort): its own translation unit
-    cmd = [hipcc()] + flags() + ["-o", tmp, src, grid] + [a for o in objs for a in ("-Xlinker", o)] + ["-lpthread"]
+    util = os.path.join(CSRC, "util.hip")  # device utilities (the streaming copy)
+    cmd = ([hipcc()] + flags() + ["-o", tmp, src, grid, util] + [a for o in objs for a in ("-Xlinker", o)] +
+           ["-lpthread"])
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1631,6 +1631,12 @@ int mrx_init(int device, mrx_ctx** out)
   return MRX_OK;
 }
 
+int mrx_device_count(void)
+{
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess ? n : 0;
+}
+
 int mrx_free(mrx_ctx* ctx)
 {
   if (!ctx) return MRX_OK;

@@ -333,3 +333,122 @@ def test_snapshot_adjoint_with_ts_matches_float64_autograd(ctx):
     print("gradient rel L2 vs float64 autograd: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
     for k, v in errs.items():
         assert v < 1e-4, k
+
+
+def _fd_directions(ctx, c, Pref, box, kf, which, base, dirs, eps):
+    """Central differences of the float64 power-spectrum loss along each direction."""
+    out = []
+    for v in dirs:
+        vals = []
+        for sgn in (1.0, -1.0):
+            pert = base + sgn * eps * v
+            dd_ = pert if which == "d" else c["d"]
+            ss_ = pert if which == "s" else c["s"]
+            vals.append(_ts_loss(ctx, c, dd_, ss_, Pref, box, kf)[0])
+        out.append((vals[0] - vals[1]) / (2 * eps))
+    return np.array(out)
+
+
+def test_snapshot_adjoint_fd_directions_256(ctx):
+    """The adjoint configuration at 256^3 (200 Mpc/h, the 512^3 cell size): the
+    hand-written gradient of the power-spectrum loss against central finite
+    differences of the GPU forward along 64 random directions in deltax and 64
+    in stars.  A deltax direction is +-1 on 8192 partially ionised cells at
+    least 1e-3 from the threshold, step 1e-3.  A stars direction is +-(the
+    cell's own stars) on 2048 partially ionised cells at least 0.1 from it, step
+    0.05: a partially ionised cell's xH is affine in its own stars and the loss
+    quadratic in xH, so the central difference is exact up to rounding for any
+    step that flips nothing, and a step of 5 % moves a cell's ratio by at most
+    5 % of itself, below its margin.  (A step of 1e-3 of the stars on 512 cells
+    moves the loss by less than the float32 forward's rounding at this size.)
+    The loss is summed in float64; the shell tables' grid means are held fixed
+    in both."""
+    N, L = 256, 200.0
+    box = L / P.HUBBLE_H
+    print("adjoint FD at 256^3: forward, backward and oracle margins", flush=True)
+    c = _ts_case(N, L, 31)
+    g0 = reion.compute_ts(ctx, c["tsp"], c["d"], c["s"], c["w"], c["hist"], c["xe0"].clone(), c["Tk0"].clone())
+    c["tsp"].fixed_globals = 1
+    c["tsp"].fixed_collapse_fraction = g0["collapse_fraction"]
+    c["tsp"].fixed_x_e_ave = g0["x_e_ave"]
+    c2 = _ts_case(N, L, 32)
+    kf = MA.k_factor_for_bins(N, 20)
+    dT2, _ = MA.snapshot(ctx, c["tsp"], c2["hp"], c2["bp"], c2["d"], c2["s"], c2["w"], c2["w"], c2["hist"],
+                         c2["xe0"], c2["Tk0"])
+    Pref = MA.power_spectrum(ctx, dT2.detach(), box, kf).detach()
+    del c2, dT2
+    loss, gd, gs, aux = _ts_loss(ctx, c, c["d"], c["s"], Pref, box, kf, grad=True)
+    xH = aux[0].cpu().numpy()
+    hpo = P.hii_params(N, L, c["hp"].redshift, efficiency=8.0, include_spin_temp=1)
+    o = O.find_hii_bubbles(hpo, c["d"].cpu().numpy(), c["s"].cpu().numpy(), c["w"].cpu().numpy(),
+                           x_e=aux[2].cpu().numpy(), want_ratio=True)
+    margin = np.abs(o["max_ratio"].astype(np.float64) - 1.0)
+    partial = (xH > 0) & (xH < 1)
+    stars_n = c["s"].cpu().numpy()[:, :, :N]
+    cand_d = np.argwhere(partial & (margin >= 1e-3))
+    cand_s = np.argwhere(partial & (margin >= 0.1) & (stars_n > 0))
+    rng = np.random.default_rng(17)
+    res = {}
+    for which, cand, K, eps, g in (("d", cand_d, 8192, 1e-3, gd), ("s", cand_s, 2048, 0.05, gs)):
+        dirs = []
+        for _ in range(64):
+            cells = cand[rng.choice(len(cand), size=K, replace=False)]
+            v = torch.zeros_like(c[which])
+            sg = torch.from_numpy(rng.choice([-1.0, 1.0], size=K).astype(np.float32)).to(v.device)
+            idx = (cells[:, 0], cells[:, 1], cells[:, 2])
+            v[idx] = sg if which == "d" else sg * c[which][idx]
+            dirs.append(v)
+        fd = _fd_directions(ctx, c, Pref, box, kf, which, c[which], dirs, eps)
+        ad = np.array([float((g * v).sum()) for v in dirs])
+        res[which] = rel_l2(ad, fd)
+        print(f"  {which}: 64 directions of {K} cells, step {eps:g}, AD vs FD rel L2 {res[which]:.2e}, median |rel| "
+              f"{np.median(np.abs(ad - fd) / np.abs(fd)):.2e}", flush=True)
+    assert res["d"] < 2e-2 and res["s"] < 2e-2
+
+
+def test_fixed_table_means_term_64(ctx):
+    """Size of what holding the shell tables' two grid means fixed drops from the
+    gradient (collapse fraction and <x_e>, ComputeTs.c:413-475, which the
+    tables depend on through tau_X): central differences of the loss with the
+    means fixed and with the forward measuring them again, along 32 deltax
+    directions at 64^3.  The difference is reported; it is bounded here at
+    5 %, its measured size being about 3 %."""
+    N, L = 64, 50.0
+    box = L / P.HUBBLE_H
+    c = _ts_case(N, L, 31)
+    g0 = reion.compute_ts(ctx, c["tsp"], c["d"], c["s"], c["w"], c["hist"], c["xe0"].clone(), c["Tk0"].clone())
+    free = c["tsp"]
+    fixed = P.ts_params(N, L, 24, c["zz"], c["fc"].copy(), 8, on_device=1)
+    fixed.fixed_globals = 1
+    fixed.fixed_collapse_fraction = g0["collapse_fraction"]
+    fixed.fixed_x_e_ave = g0["x_e_ave"]
+    c2 = _ts_case(N, L, 32)
+    kf = MA.k_factor_for_bins(N, 12)
+    dT2, _ = MA.snapshot(ctx, fixed, c2["hp"], c2["bp"], c2["d"], c2["s"], c2["w"], c2["w"], c2["hist"],
+                         c2["xe0"], c2["Tk0"])
+    Pref = MA.power_spectrum(ctx, dT2.detach(), box, kf).detach()
+    c["tsp"] = fixed
+    loss, gd, gs, aux = _ts_loss(ctx, c, c["d"], c["s"], Pref, box, kf, grad=True)
+    xH = aux[0].cpu().numpy()
+    hpo = P.hii_params(N, L, c["hp"].redshift, efficiency=8.0, include_spin_temp=1)
+    o = O.find_hii_bubbles(hpo, c["d"].cpu().numpy(), c["s"].cpu().numpy(), c["w"].cpu().numpy(),
+                           x_e=aux[2].cpu().numpy(), want_ratio=True)
+    margin = np.abs(o["max_ratio"].astype(np.float64) - 1.0)
+    cand = np.argwhere((xH > 0) & (xH < 1) & (margin >= 1e-3))
+    rng = np.random.default_rng(3)
+    dirs = []
+    for _ in range(32):
+        cells = cand[rng.choice(len(cand), size=64, replace=False)]
+        v = torch.zeros_like(c["d"])
+        v[cells[:, 0], cells[:, 1], cells[:, 2]] = torch.from_numpy(
+            rng.choice([-1.0, 1.0], size=64).astype(np.float32)).to(v.device)
+        dirs.append(v)
+    fd_fixed = _fd_directions(ctx, c, Pref, box, kf, "d", c["d"], dirs, 1e-3)
+    c["tsp"] = free
+    fd_free = _fd_directions(ctx, c, Pref, box, kf, "d", c["d"], dirs, 1e-3)
+    ad = np.array([float((gd * v).sum()) for v in dirs])
+    dropped = rel_l2(fd_fixed, fd_free)
+    print(f"fixed table means: FD fixed vs measured rel L2 {dropped:.2e}; AD vs FD fixed {rel_l2(ad, fd_fixed):.2e}, "
+          f"AD vs FD measured {rel_l2(ad, fd_free):.2e}", flush=True)
+    assert rel_l2(ad, fd_fixed) < 2e-2
+    assert dropped < 5e-2

@@ -1,0 +1,90 @@
+"""The full-snapshot configuration's chain through all 30 snapshots, z = 20 -> 6,
+in the reference's order per snapshot (dracarys.c:352-390): _ComputeTs, then
+_find_HII_bubbles with x_e as the fourth field, then
+ComputeBrightnessTemperatureBox - the HIP path against the oracle, each side
+carrying its own state (x_e, T_K, z_at_ionization, J_21_at_ionization and the
+collapse-fraction history), both fed the same SFR history ring.
+
+64^3 on a 50 Mpc/h box: the 512^3, 400 Mpc/h configuration's cell size, so
+the same shell radii and radius ladder shape.  Tolerances per snapshot: x_e,
+T_K, T_S within 1e-4 per cell; the bubble finder's parity contract (global xH
+within 1e-5, at most 1e-4 of cells flipping, each within 1e-4 of the
+threshold); delta_T within 1e-4 relative plus 1e-4 of its rms on the cells both
+sides ionise alike (T_S near T_CMB makes 1 - T_CMB/T_S small, so a relative
+bound alone would amplify T_S's rounding), <delta_T> within 1e-4.
+"""
+import numpy as np
+import pytest
+
+from meraxes_amd import params as P
+from meraxes_amd import reion
+from meraxes_amd.snapshots import SyntheticRun
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+
+
+def test_thirty_snapshot_chain_64(ctx):
+    N, L, eff = 64, 50.0, 20.0
+    run = SyntheticRun(N, L, seed=7)
+    nz = len(run.z)
+    assert nz == 30 and run.z[0] == 20.0 and run.z[-1] == 6.0
+    nh = 12  # the plan reads at most 8 snapshots back at this cell size
+    PR = 2 * (N // 2 + 1)
+    hist = np.zeros((nh, N, N, PR), np.float32)
+    fc = {"g": np.zeros(nz), "o": np.zeros(nz)}
+    st = {k: {"xe": np.zeros((N, N, PR), np.float32), "Tk": np.zeros((N, N, N), np.float32),
+              "zi": np.full((N, N, N), -1.0, np.float32), "Ji": np.zeros((N, N, N), np.float32)} for k in "go"}
+    for snap in range(nz):
+        z = float(run.z[snap])
+        d, s, f = run.inputs(snap)
+        hist = np.roll(hist, 1, axis=0)
+        hist[0] = f
+        print(f"chain snapshot {snap} z={z:.3f}", flush=True)
+        # spin temperature
+        tg = reion.compute_ts(ctx, P.ts_params(N, L, snap, run.z, fc["g"], nh), d, s, f, hist, st["g"]["xe"],
+                              st["g"]["Tk"])
+        to = O.compute_ts(P.ts_params(N, L, snap, run.z, fc["o"], nh), d, s, f, hist, st["o"]["xe"], st["o"]["Tk"])
+        st["o"]["xe"], st["o"]["Tk"] = to["x_e_box_prev"], to["Tk_box"]
+        assert tg["homogeneous"] == to["homogeneous"] == (snap == 0)
+        rx = rel(tg["x_e_box"][:, :, :N], to["x_e_box"][:, :, :N]).max()
+        rT = rel(tg["Tk_box"], to["Tk_box"]).max()
+        rS = rel(tg["TS_box"], to["TS_box"]).max()
+        assert rx <= 1e-4 and rT <= 1e-4 and rS <= 1e-4, (rx, rT, rS)
+        for k in ("volume_ave_TS", "volume_ave_TK", "volume_ave_xe"):
+            assert abs(tg[k] - to[k]) <= 1e-5 * abs(to[k]), k
+        assert fc["g"][snap] == pytest.approx(fc["o"][snap], rel=1e-9, abs=1e-15)
+        # bubble finder with x_e as the fourth field and the UVB on
+        hp = P.hii_params(N, L, z, efficiency=eff, uvb_flag=1, include_spin_temp=1)
+        og = reion.find_hii_bubbles(ctx, hp, d, s, f, x_e=tg["x_e_box"], z_at_ionization=st["g"]["zi"],
+                                    J_21_at_ionization=st["g"]["Ji"])
+        oo = O.find_hii_bubbles(hp, d, s, f, x_e=to["x_e_box"], z_at_ion=st["o"]["zi"], J21_at_ion=st["o"]["Ji"],
+                                want_ratio=True)
+        st["g"]["zi"], st["g"]["Ji"] = og["z_at_ionization"], og["J_21_at_ionization"]
+        st["o"]["zi"], st["o"]["Ji"] = oo["z_at_ionization"], oo["J_21_at_ionization"]
+        assert og["n_radii"] == oo["n_radii"]
+        flips = (og["xH"] < 1e-5) != (oo["xH"] < 1e-5)
+        margin = np.abs(oo["max_ratio"].astype(np.float64) - 1.0)
+        dxh = abs(og["volume_weighted_global_xH"] - oo["volume_weighted_global_xH"])
+        assert dxh <= 1e-5 and flips.mean() <= 1e-4
+        if flips.any():
+            assert margin[flips].max() <= 1e-4
+        # brightness temperature with T_S
+        bp = P.tb_params(N, z, include_spin_temp=1)
+        dg, ag = reion.brightness_temp(ctx, bp, og["xH"], d, tg["TS_box"])
+        do, ao = O.brightness_temp(bp, oo["xH"], d, to["TS_box"])
+        keep = ~flips
+        rms = float(np.sqrt(np.mean(do.astype(np.float64) ** 2)))
+        dd = np.abs(dg[keep].astype(np.float64) - do[keep])
+        assert (dd <= 1e-4 * np.abs(do[keep]) + 1e-4 * rms).all()
+        assert abs(ag - ao) <= 1e-4 * abs(ao) + 1e-6 * rms
+        print(f"  x_e {rx:.1e} T_K {rT:.1e} T_S {rS:.1e} | <xH> {og['volume_weighted_global_xH']:.6f} "
+              f"(|d| {dxh:.1e}, flips {int(flips.sum())}) | <dTb> gpu {ag:.5f} oracle {ao:.5f}", flush=True)
+    # the run reionises part of the box and heats the IGM above the CMB by z = 6
+    assert og["volume_weighted_global_xH"] < 0.99 and tg["volume_ave_TK"] > 2.728 * 7

@@ -120,3 +120,83 @@ def test_bubble_finder_invariants_512(ctx, fields):
     for k in ("xH", "r_bubble", "J_21", "z_at_ionization", "J_21_at_ionization"):
         assert torch.equal(o3[k], o1[k]), k
     assert o3["volume_weighted_global_xH"] == o1["volume_weighted_global_xH"]
+
+
+def test_spin_temperature_and_brightness_temperature_512(ctx, fields):
+    """The full-snapshot configuration's Ts and delta_T at 512^3, 400 Mpc/h:
+    snapshot 0 (z = 20 = ReionMaxHeatingRedshift) takes the homogeneous branch,
+    where x_e and T_K equal the RECFAST splines at z everywhere
+    (ComputeTs.c:189-262; values from the oracle's restatement of the table);
+    snapshot 24 (z = 8.41, 40 shells, an 8-slot SFR history) reports averages
+    that are the means of its own grids (ComputeTs.c:940-985), keeps x_e in
+    [0, 1) and T_S, T_K positive; then the bubble finder with x_e as the fourth
+    field, and delta_T = C xH (1 + delta) (1 - T_CMB (1+z) / T_S)
+    (BrightnessTemperature.c:46-124) recomputed cell by cell in float64 from
+    the GPU's own grids, <delta_T> their mean."""
+    import torch
+
+    from oracle import ts_host
+
+    d, s, w = fields
+    zz = P.snapshot_redshifts()
+    nh = 8
+    fc = np.zeros(len(zz))
+    fc[:24] = np.linspace(1e-4, 5e-3, 24)
+    xe_prev = torch.zeros_like(d)
+    Tk = torch.zeros((N, N, N), device=d.device)
+    # snapshot 0: homogeneous
+    print("full size: spin temperature, homogeneous branch", flush=True)
+    hist = torch.zeros((nh,) + tuple(w.shape), device=d.device)
+    p0 = P.ts_params(N, L, 0, zz, fc, nh, on_device=1)
+    g0 = reion.compute_ts(ctx, p0, d, s, w, hist, xe_prev, Tk)
+    torch.cuda.synchronize()
+    h = ts_host.host(p0)
+    assert g0["homogeneous"] == 1
+    assert (xe_prev[:, :, :N] == np.float32(h.xion_RECFAST(zz[0]))).all()
+    assert (Tk == np.float32(h.T_RECFAST(zz[0]))).all()
+    TS0 = g0["TS_box"].double()
+    assert torch.isfinite(TS0).all() and (TS0 > 0).all()
+    assert abs(g0["volume_ave_TS"] - TS0.mean().item()) <= 1e-9 * TS0.mean().item()
+    # snapshot 24: inhomogeneous, 40 shells
+    print("full size: spin temperature, 40 shells", flush=True)
+    snap = 24
+    z = float(zz[snap])
+    for k in range(nh):
+        hist[k] = w * float(((1 + z) / (1 + zz[max(snap - k, 0)])) ** 3)
+    xe_prev.fill_(2e-3)
+    Tk.fill_(30.0)
+    p1 = P.ts_params(N, L, snap, zz, fc, nh, on_device=1)
+    g1 = reion.compute_ts(ctx, p1, d, s, w, hist, xe_prev, Tk)
+    torch.cuda.synchronize()
+    assert g1["homogeneous"] == 0 and g1["no_light"] == 0
+    xe = g1["x_e_box"][:, :, :N].double()
+    TK, TS = g1["Tk_box"].double(), g1["TS_box"].double()
+    assert torch.equal(g1["x_e_box"], xe_prev)
+    assert ((xe >= 0) & (xe < 1)).all() and (TK > 0).all() and (TS > 0).all() and torch.isfinite(TS).all()
+    for key, grid in (("volume_ave_TS", TS), ("volume_ave_TK", TK), ("volume_ave_xe", xe)):
+        assert abs(g1[key] - grid.mean().item()) <= 1e-9 * abs(grid.mean().item()), key
+    assert TK.mean().item() > 30.0  # X-ray heating outruns adiabatic cooling over this step
+    # bubbles with x_e, then delta_T from the GPU's own grids
+    print("full size: bubble finder with x_e and the brightness temperature", flush=True)
+    hp = P.hii_params(N, L, z, efficiency=20.0, uvb_flag=1, include_spin_temp=1)
+    o = reion.find_hii_bubbles(ctx, hp, d, s, w, x_e=g1["x_e_box"])
+    bp = P.tb_params(N, z, include_spin_temp=1)
+    dT, ave = reion.brightness_temp(ctx, bp, o["xH"], d, g1["TS_box"])
+    torch.cuda.synchronize()
+    # the constant as the reference forms it (BrightnessTemperature.c:46-52): float
+    # cosmology, the products of floats in float, the rest in double
+    Om, h_ = np.float32(P.OMEGA_M), np.float32(P.HUBBLE_H)
+    Ob = np.float32(float(Om) * P.BARYON_FRAC)
+    zf = float(np.float32(z))
+    obh2 = float(np.float32(np.float32(Ob * h_) * h_))
+    C = float(np.float32(27.0 * (obh2 / 0.023) * np.sqrt((0.15 / float(Om) / float(h_) / float(h_)) * (1.0 + zf)
+                                                       / 10.0)))
+    want = C * o["xH"].double() * (1 + d[:, :, :N].double()) * (1 - 2.728 * (1 + zf) / TS)
+    got = dT.double()
+    err = (got - want).abs() / want.abs().clamp_min(1e-30)
+    big = want.abs() > 1e-3 * want.abs().max()
+    assert err[big].max().item() < 1e-6
+    assert (got - want).abs().max().item() < 1e-6 * want.abs().max().item()
+    assert abs(ave - got.mean().item()) <= 1e-9 * abs(got.mean().item())
+    del hist, xe_prev, Tk, o, dT
+    torch.cuda.empty_cache()

