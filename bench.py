@@ -14,7 +14,7 @@ kernel launch on the stream the kernels run on, the dominant kernel's average
 launch time and its achieved HBM bandwidth against the MI355X peak (and its
 PMC-measured bytes per launch where a summary under profiles/ holds them), the
 per-kernel table, the device time per stage, the path's fraction of peak on
-the 3-pass byte model, a 1 GiB device-copy ceiling, and a CPU baseline: the
+the 3-pass byte model, a 2 GiB device-copy ceiling, and a CPU baseline: the
 oracle (the C restatement of the reference's FFTW-float path) on the host cores
 over a bounded sample of the same workload.  Unless --no-extras is given the
 snapshot run also times the hand-written adjoint against its forward at 256^3
@@ -183,10 +183,11 @@ def cpu_baseline(snapshot):
     }
 
 
-def copy_ceiling(torch, ctx, nbytes=1 << 30, reps=10):
-    """Practical HBM ceilings for a 1 GiB device-to-device copy (bytes read + written
-    per second): the library's streaming copy kernel (16-byte loads, four in flight
-    per thread, non-temporal stores; HIP events on its stream) and torch's copy_."""
+def copy_ceiling(torch, ctx, nbytes=2 << 30, reps=10):
+    """Practical HBM ceilings for a 2 GiB device-to-device copy (bytes read + written
+    per second): the library's streaming copy kernel (one 16-byte load and one
+    non-temporal store per thread over a grid covering the buffer; HIP events on
+    its stream) and torch's copy_."""
     import ctypes as C
 
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
@@ -208,7 +209,7 @@ def copy_ceiling(torch, ctx, nbytes=1 << 30, reps=10):
     t = e0.elapsed_time(e1) / 1e3 / reps
     del src, dst
     return {"GB_s": 2 * nbytes / t_stream / 1e9, "frac_of_peak": 2 * nbytes / t_stream / HBM_PEAK, "bytes": nbytes,
-            "kernel": "mrx_stream_copy (1 GiB, 10 repetitions)",
+            "kernel": "mrx_stream_copy (2 GiB, 10 repetitions)",
             "torch_copy_GB_s": 2 * nbytes / t / 1e9, "torch_copy_frac_of_peak": 2 * nbytes / t / HBM_PEAK}
 
 

@@ -508,8 +508,9 @@ int mrx_construct_baryon_grids(mrx_ctx* ctx, const mrx_grid_params* p, long n_ga
 /* ---------------------------------------------------------------------------
  * Device copy rate.  No counterpart in the reference: copies `bytes` (a
  * multiple of 16, 16-byte aligned device buffers) from src to dst reps times
- * on the context stream with a streaming kernel (16-byte loads, several in
- * flight per thread, non-temporal stores) after one warm-up copy, and writes
+ * on the context stream with a streaming kernel (one 16-byte load and one
+ * non-temporal store per thread, a grid covering the buffer) after one
+ * warm-up copy, and writes
  * the average milliseconds per copy measured with HIP events on that stream.
  * For callers (the benchmark) that set achieved bandwidth beside the device's
  * practical copy rate. */

@@ -187,6 +187,10 @@ bool supported_n(int N)
     case 256:
     case 384:
     case 512:
+    case 160:
+    case 320:
+    case 640:
+    case 1024:
       return true;
     default:
       return false;
@@ -563,7 +567,7 @@ struct Pipe {
     MRX_TRY(check_launch(ctx, "k_window_table"));
     {
       PROF("ix_filter_k");
-      k_ix_filter_k<N><<<N * (NH / Cfg<N>::TW), Cfg<N>::COL_NT, 0, ctx->stream>>>(
+      k_ix_filter_k<N><<<N * (NH / Cfg<N>::TW) * K, Cfg<N>::COL_NT, 0, ctx->stream>>>(
         U, V, TK, ctx->tw, ctx->wtab, nt, nc, (long)N * N, tstride, nfield, K);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter_k"));
@@ -579,19 +583,41 @@ struct Pipe {
   static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp, int pitch = N + 2,
                    int kzlive = NH)
   {
-    const dim3 gz(N * N / Cfg<N>::RZ, nfield);
     PROF("c2r_z");
+    if constexpr (N == 256 || N == 512) {
+      // the row transforms in registers
+      constexpr int A = N / 32;
+      k_c2r_z_reg<A><<<dim3(N * N / (256 / A), nfield), 256, 0, ctx->stream>>>(T, out, ctx->tw, nc,
+                                                                              (long)N * N * pitch, clamp, pitch, kzlive);
+      return check_launch(ctx, "k_c2r_z_reg");
+    }
+    const dim3 gz(N * N / Cfg<N>::RZ, nfield);
     k_c2r_z<N><<<gz, kNT, 0, ctx->stream>>>(T, out, ctx->tw, nc, (long)N * N * pitch, clamp, pitch, kzlive);
     return check_launch(ctx, "k_c2r_z");
   }
 
   // the ladder's last pass; fields deltax, stars, sfr (+ x_e) (+ N_rec when rec)
+  // 256^3 and 512^3 without recombinations: the row transforms in registers
+  // (k_c2r_threshold_reg); otherwise staged in LDS (k_c2r_threshold)
+  static constexpr bool kRegThr = (N == 256 || N == 512);
   static int threshold(mrx_ctx* ctx, const float2* Tsrc, const float* deltax, float* xH, float* rb, float* J21,
                        float* zin, float* J21i, float* Gamma12, bool rec, const StepArgs& a)
   {
-    const int nb = N * N / ThrCfg<N>::RZ;
-    constexpr int NT = ThrCfg<N>::NT;
+    const int nb = nblocks_threshold(rec);
     PROF("c2r_threshold");
+    if constexpr (kRegThr) {
+      if (!rec) {
+        constexpr int A = N / 32;
+        if (a.nfield == 4)
+          k_c2r_threshold_reg<A, 4><<<nb, 256, 0, ctx->stream>>>(Tsrc, nc, ctx->tw, deltax, ctx->state, xH, rb, J21,
+                                                                zin, J21i, ctx->Rtab, ctx->partials, a);
+        else
+          k_c2r_threshold_reg<A, 3><<<nb, 256, 0, ctx->stream>>>(Tsrc, nc, ctx->tw, deltax, ctx->state, xH, rb, J21,
+                                                                zin, J21i, ctx->Rtab, ctx->partials, a);
+        return check_launch(ctx, "k_c2r_threshold_reg");
+      }
+    }
+    constexpr int NT = ThrCfg<N>::NT;
 #define MRX_THR(NF_, REC_)                                                                                          \
   k_c2r_threshold<N, NF_, REC_><<<nb, NT, 0, ctx->stream>>>(Tsrc, nc, ctx->tw, deltax, ctx->state, xH, rb, J21,   \
                                                            zin, J21i, Gamma12, ctx->Rtab, ctx->partials, a)
@@ -607,7 +633,13 @@ struct Pipe {
     return check_launch(ctx, "k_c2r_threshold");
   }
 
-  static int nblocks_threshold() { return N * N / ThrCfg<N>::RZ; }
+  static int nblocks_threshold(bool rec)
+  {
+    if constexpr (kRegThr) {
+      if (!rec) return N * N / RegThr<N / 32>::RWG;
+    }
+    return N * N / ThrCfg<N>::RZ;
+  }
 
   static int pack_herm(mrx_ctx* ctx, const float2* H, float2* U)
   {
@@ -645,6 +677,10 @@ int dispatch_n(int N, A&&... args)
     case 256: return F<256>::run(args...);
     case 384: return F<384>::run(args...);
     case 512: return F<512>::run(args...);
+    case 160: return F<160>::run(args...);
+    case 320: return F<320>::run(args...);
+    case 640: return F<640>::run(args...);
+    case 1024: return F<1024>::run(args...);
     default: return MRX_EUNSUPPORTED;
   }
 }
@@ -736,7 +772,7 @@ struct HiiRun {
       const int S = ladder(p, R);
       if (S < 1) return set_err(ctx, MRX_EINVAL, "radius ladder is empty");
       if (S > kMaxSteps) return set_err(ctx, MRX_EUNSUPPORTED, "radius ladder has %d steps (max %d)", S, kMaxSteps);
-      MRX_TRY(ensure_partials(ctx, (size_t)P::nblocks_threshold() * 4 + 8 + kRedBlocks * kRedMaxV));
+      MRX_TRY(ensure_partials(ctx, (size_t)P::nblocks_threshold(rec) * 4 + 8 + kRedBlocks * kRedMaxV));
       if (!ctx->Rtab) HIP_TRY(hipMalloc(&ctx->Rtab, sizeof(float) * 256));
       if (!ctx->sums) HIP_TRY(hipMalloc(&ctx->sums, sizeof(double) * 8));
       std::vector<float> Rf(S);
@@ -824,7 +860,7 @@ struct HiiRun {
         }
         MRX_TRY(check_launch(ctx, "k_nrec_update"));
       }
-      const int nb = P::nblocks_threshold();
+      const int nb = P::nblocks_threshold(rec);
       {
         PROF("reduce_partials");
         MRX_TRY(reduce_partials(ctx, ctx->stream, ctx->partials, nb, 4, 1, ctx->partials + (size_t)nb * 4 + 8,

