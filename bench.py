@@ -89,6 +89,18 @@ def kernel_bytes(name, N, F, radii=1.0):
     return table.get(name)
 
 
+def path_bytes_hii(N, F, S):
+    """The bubble finder's algorithmic bytes for one box, the sum of its kernels'
+    (kernel_bytes): F forward transforms, then per radius the windowed inverse x
+    pass, the inverse y pass and the z pass + threshold; the window tables
+    (shared by the boxes of a batch cohort) are left out."""
+    spec = 8 * N * N * (N // 2)
+    fwd = F * (kernel_bytes("r2c_z", N, F) + 2 * 2 * spec)
+    per_radius = kernel_bytes("ix_filter", N, F) + kernel_bytes("colpass_y_inv", N, F) + kernel_bytes(
+        "c2r_threshold", N, F)
+    return fwd + S * per_radius
+
+
 def cpu_info():
     """The host's CPU model, its logical CPUs and the CPUs this process may use."""
     model = "unknown"
@@ -553,8 +565,13 @@ def bench_batched(ctx, steps, warmup, per_size=BATCH_PER_SIZE):
         torch.cuda.synchronize()
         tp = time.perf_counter() - t0
         bs.close()
-        per_size[str(N)] = {"batched_ms": tb * 1e3, "plain_loop_ms": tp * 1e3, "speedup": tp / tb}
-        log(f"[bench] batched {N}^3 x {len(sub)}: {tb * 1e3:.2f} ms, plain loop {tp * 1e3:.2f} ms, x{tp / tb:.2f}")
+        alg = sum(path_bytes_hii(N, 3, len(P.radius_ladder(b["params"]))) for b in sub)
+        per_size[str(N)] = {"batched_ms": tb * 1e3, "plain_loop_ms": tp * 1e3, "speedup": tp / tb,
+                            "alg_bytes": alg, "batched_GB_s": alg / tb / 1e9, "batched_frac": alg / tb / HBM_PEAK,
+                            "plain_loop_GB_s": alg / tp / 1e9, "plain_loop_frac": alg / tp / HBM_PEAK}
+        log(f"[bench] batched {N}^3 x {len(sub)}: {tb * 1e3:.2f} ms ({alg / tb / 1e9:.0f} GB/s, "
+            f"{alg / tb / HBM_PEAK:.2f} of peak), plain loop {tp * 1e3:.2f} ms ({alg / tp / 1e9:.0f} GB/s, "
+            f"{alg / tp / HBM_PEAK:.2f}), x{tp / tb:.2f}")
     return {
         "metric": "cells*radii/s of find_HII_bubbles over 96 uneven boxes, batched (one graph launch)",
         "value": cells_radii / t_batch,

@@ -45,7 +45,9 @@ typedef struct mrx_ctx mrx_ctx;
  * are read from run_globals.params / .physics / .units; here they are explicit.
  */
 typedef struct mrx_hii_params {
-  int grid_dim;          /* ReionGridDim (N); the whole box lives on this one GPU          */
+  int grid_dim;          /* ReionGridDim (N); the whole box lives on this one GPU.  N is  */
+                         /* one of 32 48 64 96 128 160 192 256 320 384 512 640 1024;      */
+                         /* others return MRX_EUNSUPPORTED (every entry point taking N)  */
   int filter_type;       /* ReionFilterType, MRX_FILTER_*                                   */
   int rtom_filter_type;  /* ReionRtoMFilterType: 0 top-hat, 1 Gaussian (RtoM, :26-45)     */
   int uvb_flag;          /* ReionUVBFlag: write J_21 and J_21_at_ionization                 */

@@ -912,9 +912,15 @@ struct ThrCfg {
 // cell's clamped filtered fields (deltax, stars, weighted_sfr, then x_e when
 // there are four besides N_rec, then N_rec when REC); st: its state byte.  The
 // first crossing writes the cell's J_21 (and Gamma_12 with REC); on the last
-// step xh, rb and jflag (1 where the cell's J_21 is kept) are set.  zcheck:
-// whether z_at_ionization is to be set if still unset, jaux the J_21 it takes.
-// Returns the new state byte.
+// step xh, rb and jflag (1 where the cell's J_21 is kept) are set, and zcheck
+// says whether z_at_ionization is to be set if still unset (:427-434), with
+// J_21_at_ionization = jaux * bias.  The reference sets them at the cell's
+// first crossing, to the snapshot's redshift and that crossing's J_21; the
+// same values are set here at the last step, where every cell that crossed
+// during the snapshot is ionised and J21 still holds its first crossing's
+// J_21: for those (jflag = 1) the caller takes jaux from J21, read after this
+// call.  Deferring the check keeps a dependent load of z_at_ionization out of
+// every ladder step.  Returns the new state byte.
 template <int NF, bool REC>
 __device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, long ir, const StepArgs& a,
                                                    float* __restrict__ J21, float* __restrict__ Gamma12,
@@ -949,12 +955,12 @@ __device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, 
   auto j21_aux = [&]() { return a.uvb ? (float)((double)v[2] / a.pixel_volume * a.J21c) : 0.0f; };
   const bool never = (st == 0xFFu);
   zcheck = false;
+  jaux = 0.0f;
   uint32_t st_new = st;
   if (cross) {
     if (never) {
       if (a.uvb) J21[ir] = j21_aux();
       if (REC) Gamma12[ir] = (float)(a.gamma_pref * ((double)v[2] / a.pixel_volume));  // :398-404
-      zcheck = true;
     }
     st_new = (uint32_t)a.step;
   }
@@ -968,12 +974,12 @@ __device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, 
         xh = 0.f;
       else if (xh > 1.0f)
         xh = 1.f;
-      if (xh < kRelTol) zcheck = true;
+      if (xh < kRelTol) jaux = j21_aux();
     }
+    zcheck = xh < kRelTol;  // every ionised cell, and the partial ones below REL_TOL
     rb = ion ? Rtab[cross ? a.step : (int)st] : 0.0f;
     jflag = ion ? 1.0f : 0.0f;
   }
-  jaux = zcheck ? j21_aux() : 0.0f;
   return st_new;
 }
 
@@ -1094,14 +1100,14 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
         if (a.uvb) s_J += (double)jo[q];
       }
       if (a.uvb) *reinterpret_cast<float4*>(J21 + ir0) = make_float4(jo[0], jo[1], jo[2], jo[3]);
-    }
-    if (zcheck4) {
+      if (zcheck4) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const long ir = ir0 + q;
-        if (((zcheck4 >> q) & 1) && z_in[ir] < 0) {
-          z_in[ir] = (float)a.redshift;
-          if (a.uvb) J21_ion[ir] = jaux4[q] * (float)a.bias;
+        for (int q = 0; q < 4; ++q) {
+          const long ir = ir0 + q;
+          if (((zcheck4 >> q) & 1) && z_in[ir] < 0) {
+            z_in[ir] = (float)a.redshift;
+            if (a.uvb) J21_ion[ir] = ((j4[q] != 0.0f) ? jj[q] : jaux4[q]) * (float)a.bias;
+          }
         }
       }
     }
@@ -1321,26 +1327,26 @@ __global__ __launch_bounds__(256) void k_c2r_threshold_reg(const float2* __restr
         if (a.uvb) s_J += (double)jo[e];
       }
       if (a.uvb) *reinterpret_cast<float2*>(J21 + ir0) = make_float2(jo[0], jo[1]);
-    }
+      if (zc2[0] || zc2[1]) {
+        const float2 zi = *reinterpret_cast<const float2*>(z_in + ir0);
+        const float zi2[2] = {zi.x, zi.y};
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      if (zc2[e] && z_in[ir0 + e] < 0) {
-        z_in[ir0 + e] = (float)a.redshift;
-        if (a.uvb) J21_ion[ir0 + e] = ja2[e] * (float)a.bias;
+        for (int e = 0; e < 2; ++e) {
+          if (zc2[e] && zi2[e] < 0) {
+            z_in[ir0 + e] = (float)a.redshift;
+            if (a.uvb) J21_ion[ir0 + e] = ((j2[e] != 0.0f) ? jj[e] : ja2[e]) * (float)a.bias;
+          }
+        }
       }
     }
   }
   if (a.last) {
-    double* p = partials + (long)blockIdx.x * 4;
-    double v0 = block_sum<256>(s_xH, red);
-    double v1 = block_sum<256>(s_xHm, red);
-    double v2 = block_sum<256>(s_m, red);
-    double v3 = block_sum<256>(s_J, red);
+    double v[4] = {s_xH, s_xHm, s_m, s_J};
+    block_sum_n<256, 4>(v, red);
     if (tid == 0) {
-      p[0] = v0;
-      p[1] = v1;
-      p[2] = v2;
-      p[3] = v3;
+      double* p = partials + (long)blockIdx.x * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) p[k] = v[k];
     }
   }
 }

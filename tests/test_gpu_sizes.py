@@ -17,6 +17,7 @@ import pytest
 
 from meraxes_amd import params as P
 from meraxes_amd import reion, synth
+from meraxes_amd._lib import Context
 from oracle import oracle as O
 
 from test_gpu_hii import check_parity, run_pair
@@ -127,10 +128,23 @@ def test_spin_temperature_160(ctx):
             assert r <= 1e-4
 
 
-@pytest.mark.parametrize("N,L", [(640, 500.0), (1024, 800.0)])
-def test_large_grid_properties(ctx, N, L):
+@pytest.fixture
+def own_ctx():
+    """A context of its own, released after the test: a 1024^3 box's buffers
+    (spectra, shells) are tens of GiB, which the session's context would keep."""
     import torch
 
+    c = Context(0)
+    yield c
+    c.close()
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("N,L", [(640, 500.0), (1024, 800.0)])
+def test_large_grid_properties(own_ctx, N, L):
+    import torch
+
+    ctx = own_ctx
     print(f"{N}^3: generating inputs on the GPU", flush=True)
     d, s, w = synth.snapshot_inputs_torch(N, 5, "cuda")
     torch.cuda.synchronize()
@@ -181,3 +195,40 @@ def test_large_grid_properties(ctx, N, L):
     print(f"{N}^3: <xH>_V {o['volume_weighted_global_xH']:.6f}, ionised {ion.double().mean().item():.4f}",
           flush=True)
     assert 0.0 < o["volume_weighted_global_xH"] < 1.0
+
+
+def test_spin_temperature_1024(own_ctx):
+    """The spin-temperature step at 1024^3 (800 Mpc/h, the 512^3 / 400 Mpc/h
+    cell size): 40 shells of 4 GiB each stay resident (160 GiB) beside the
+    step's grids on one GPU.  Snapshot 24 with an 8-slot SFR history: the
+    averages are the means of the step's own grids (ComputeTs.c:940-985), x_e
+    stays in [0, 1), T_S and T_K positive and finite, and X-ray heating raises
+    the mean T_K over the step, as at 512^3."""
+    import torch
+
+    ctx = own_ctx
+    N, L = 1024, 800.0
+    print(f"{N}^3: spin temperature, generating inputs on the GPU", flush=True)
+    d, s, w = synth.snapshot_inputs_torch(N, 6, "cuda")
+    zz = P.snapshot_redshifts()
+    nh, snap = 8, 24
+    z = float(zz[snap])
+    fc = np.zeros(len(zz))
+    fc[:snap] = np.linspace(1e-4, 5e-3, snap)
+    hist = torch.empty((nh,) + tuple(w.shape), device=w.device)
+    for k in range(nh):
+        hist[k] = w * float(((1 + z) / (1 + zz[max(snap - k, 0)])) ** 3)
+    xe_prev = torch.full_like(d, 2e-3)
+    Tk = torch.full((N, N, N), 30.0, device=d.device)
+    print(f"{N}^3: spin temperature, 40 shells", flush=True)
+    g = reion.compute_ts(ctx, P.ts_params(N, L, snap, zz, fc, nh, on_device=1), d, s, w, hist, xe_prev, Tk)
+    torch.cuda.synchronize()
+    assert g["homogeneous"] == 0 and g["no_light"] == 0
+    xe = g["x_e_box"][:, :, :N].double()
+    TK, TS = g["Tk_box"].double(), g["TS_box"].double()
+    assert ((xe >= 0) & (xe < 1)).all() and (TK > 0).all() and (TS > 0).all() and torch.isfinite(TS).all()
+    for key, grid in (("volume_ave_TS", TS), ("volume_ave_TK", TK), ("volume_ave_xe", xe)):
+        assert abs(g[key] - grid.mean().item()) <= 1e-9 * abs(grid.mean().item()), key
+    print(f"{N}^3: <T_S> {g['volume_ave_TS']:.4f} <T_K> {g['volume_ave_TK']:.4f} <x_e> {g['volume_ave_xe']:.3e}, "
+          f"device memory in use {torch.cuda.memory_allocated() / 2**30:.1f} GiB (torch)", flush=True)
+    assert TK.mean().item() > 30.0
