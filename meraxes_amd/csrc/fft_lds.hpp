@@ -17,6 +17,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Multiply-adds are fused only within an expression (a * b + c as written),
+// never across statements: the fusion the compiler may otherwise choose
+// depends on the code a transform is inlined into, and then the same
+// transform rounds differently in different kernels.
+#pragma clang fp contract(on)
+
 namespace mrx {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }

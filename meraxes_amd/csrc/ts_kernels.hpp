@@ -257,6 +257,21 @@ struct ShellSums {
   double hf, hs, xf, xs, lf, ls, st;
 };
 
+// shell ct's terms of the seven sums; row: the coefficients at the cell's
+// x_e bracket, cst: the stellar Ly-alpha ones; v: the shell's (clamped) value
+__device__ __forceinline__ void shell_term(ShellSums& r, const double2* row, const double* cst, int ct, float vf)
+{
+  const double v = (double)vf;
+  const double2 a = row[3 * ct], b = row[3 * ct + 1], c = row[3 * ct + 2];
+  r.hf = fma(v, a.x, r.hf);
+  r.xf = fma(v, a.y, r.xf);
+  r.lf = fma(v, b.x, r.lf);
+  r.hs = fma(v, b.y, r.hs);
+  r.xs = fma(v, c.x, r.xs);
+  r.ls = fma(v, c.y, r.ls);
+  r.st = fma(v, cst[ct], r.st);
+}
+
 __device__ __forceinline__ ShellSums ts_shell_sums(const TsDev& t, const double* tabs, int lo, const float* shells,
                                                    long ncell, long i)
 {
@@ -283,15 +298,7 @@ __device__ __forceinline__ ShellSums ts_shell_sums(const TsDev& t, const double*
     for (int u = 0; u < kG; ++u) {
       const int ct = c0 + u;
       if (ct >= ns) break;
-      const double v = (double)cur[u];
-      const double2 a = row[3 * ct], b = row[3 * ct + 1], c = row[3 * ct + 2];
-      r.hf = fma(v, a.x, r.hf);
-      r.xf = fma(v, a.y, r.xf);
-      r.lf = fma(v, b.x, r.lf);
-      r.hs = fma(v, b.y, r.hs);
-      r.xs = fma(v, c.x, r.xs);
-      r.ls = fma(v, c.y, r.ls);
-      r.st = fma(v, cst[ct], r.st);
+      shell_term(r, row, cst, ct, cur[u]);
     }
   }
   return r;
@@ -312,6 +319,57 @@ __device__ __forceinline__ int ts_bracket(const TsDev& t, float xe0, double* fr)
   return lo;
 }
 
+// The update of one cell (ComputeTs.c:743-900) from its shell sums S (unused
+// when no_light): the new x_e, T_K and T_S, and the cell's terms of the seven
+// block sums (J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e).  dl, xe0, T: the
+// cell's delta, previous x_e and T_K; fr: its x_e bracket offset.
+__device__ __forceinline__ void ts_cell_update(const TsDev& t, const ShellSums& S, float dl, float xe0, double T,
+                                               double fr, float& xn_out, float& Tn_out, float& ts_out,
+                                               double (&sum)[7])
+{
+  const double x_e = xe0;
+  const double n_b = t.NBP3 * (1 + dl);
+  double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
+  if (!t.no_light) {
+    dxheat = (S.hf + fr * S.hs) * t.const_zp_prefactor;
+    dxion = (S.xf + fr * S.xs) * t.const_zp_prefactor;
+    dxlya = (S.lf + fr * S.ls) * (t.const_zp_prefactor * n_b);
+    dstarlya = S.st * ((2.9979e10 / (4. * M_PI)) / (1.6726e-24 / 1.989e33));
+  }
+  const double sink = dev_alpha_A(T) * 2.0 * x_e * x_e * t.f_H * n_b;
+  const double dxe_dzp = t.dt_dzp * (dxion - sink);
+  double dadia = t.dadia0;
+  if (fabs(dl) > 1e-7) dadia += t.dgrowth / (1.0 / dl + t.growth);
+  dadia *= (2.0 / 3.0) * T;
+  const double dspec = -dxe_dzp * T / (1 + x_e);
+  double dcomp = (-1.51e-4) * (x_e / (1.0 + x_e + t.f_He)) / t.HH / t.hubble_h * t.T4 / t.one_p_z;
+  dcomp *= t.Trad - T;
+  const double dxheat_dzp = dxheat * t.dt_dzp * 2.0 / 3.0 / 1.3806e-16 / (1.0 + x_e);
+  const double der1 = dxheat_dzp + dcomp + dspec + dadia;
+  const double der2 = dxlya + dstarlya;
+  const double der4 = t.dt_dzp * dxion;
+  float xn = (float)((double)xe0 + dxe_dzp * t.dzp);
+  if (xn > 1)
+    xn = (float)(1 - 1e-7);
+  else if (xn < 0)
+    xn = 0;
+  float Tn = (float)T;
+  if (Tn < (float)5e4) Tn = (float)((double)Tn + der1 * t.dzp);
+  if (Tn < 0) Tn = (float)t.Tk_floor;
+  float xa;
+  const float ts = dev_get_Ts(t, dl, Tn, xn, (float)der2, &xa);
+  xn_out = xn;
+  Tn_out = Tn;
+  ts_out = ts;
+  sum[0] += der2;
+  sum[1] += xa;
+  sum[2] += dxheat_dzp;
+  sum[3] += der4;
+  sum[4] += ts;
+  sum[5] += Tn;
+  sum[6] += xn;
+}
+
 // One cell per thread (a grid-stride loop keeps the snapshot constants live
 // across iterations and halves the occupancy).
 __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, const float* __restrict__ shells,
@@ -325,7 +383,7 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
   load_ts_coef(t, tabs);
   __syncthreads();
   const long ncell = (long)N * N * N;
-  double sJ = 0, sxa = 0, sXh = 0, sXi = 0, sTS = 0, sTK = 0, sxe = 0;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
   const long i = (long)blockIdx.x * kNT + threadIdx.x;
   if (i < ncell) {
     const long r = i / N;
@@ -333,53 +391,18 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
     const long ip = r * (N + 2) + z;
     const float dl = deltax[ip];
     const float xe0 = xe_prev[ip];
-    const double x_e = xe0, T = Tk[i];
+    const double T = Tk[i];
     double fr;
     const int lo = ts_bracket(t, xe0, &fr);
-    const double n_b = t.NBP3 * (1 + dl);
-    double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
-    if (!t.no_light) {
-      const ShellSums S = ts_shell_sums(t, tabs, lo, shells, ncell, i);
-      dxheat = (S.hf + fr * S.hs) * t.const_zp_prefactor;
-      dxion = (S.xf + fr * S.xs) * t.const_zp_prefactor;
-      dxlya = (S.lf + fr * S.ls) * (t.const_zp_prefactor * n_b);
-      dstarlya = S.st * ((2.9979e10 / (4. * M_PI)) / (1.6726e-24 / 1.989e33));
-    }
-    const double sink = dev_alpha_A(T) * 2.0 * x_e * x_e * t.f_H * n_b;
-    const double dxe_dzp = t.dt_dzp * (dxion - sink);
-    double dadia = t.dadia0;
-    if (fabs(dl) > 1e-7) dadia += t.dgrowth / (1.0 / dl + t.growth);
-    dadia *= (2.0 / 3.0) * T;
-    const double dspec = -dxe_dzp * T / (1 + x_e);
-    double dcomp = (-1.51e-4) * (x_e / (1.0 + x_e + t.f_He)) / t.HH / t.hubble_h * t.T4 / t.one_p_z;
-    dcomp *= t.Trad - T;
-    const double dxheat_dzp = dxheat * t.dt_dzp * 2.0 / 3.0 / 1.3806e-16 / (1.0 + x_e);
-    const double der1 = dxheat_dzp + dcomp + dspec + dadia;
-    const double der2 = dxlya + dstarlya;
-    const double der4 = t.dt_dzp * dxion;
-    float xn = (float)((double)xe0 + dxe_dzp * t.dzp);
-    if (xn > 1)
-      xn = (float)(1 - 1e-7);
-    else if (xn < 0)
-      xn = 0;
-    float Tn = (float)T;
-    if (Tn < (float)5e4) Tn = (float)((double)Tn + der1 * t.dzp);
-    if (Tn < 0) Tn = (float)t.Tk_floor;
-    float xa;
-    const float ts = dev_get_Ts(t, dl, Tn, xn, (float)der2, &xa);
+    ShellSums S = {0, 0, 0, 0, 0, 0, 0};
+    if (!t.no_light) S = ts_shell_sums(t, tabs, lo, shells, ncell, i);
+    float xn, Tn, ts;
+    ts_cell_update(t, S, dl, xe0, T, fr, xn, Tn, ts, v);
     xe_prev[ip] = xn;
     Tk[i] = Tn;
     TS[i] = ts;
-    sJ = der2;
-    sxa = xa;
-    sXh = dxheat_dzp;
-    sXi = der4;
-    sTS = ts;
-    sTK = Tn;
-    sxe = xn;
   }
   double* p = partials + (long)blockIdx.x * 7;
-  double v[7] = {sJ, sxa, sXh, sXi, sTS, sTK, sxe};
   block_sum_n<kNT, 7>(v, red);
   if (threadIdx.x == 0) {
 #pragma unroll
