@@ -930,12 +930,12 @@ struct C2R {
 void fill_kappa(KappaSpline& k, const mrx_spline& s)
 {
   k.n = s.n;
-  for (int i = 0; i < s.n && i < 27; ++i) {
-    k.x[i] = s.x[i];
-    k.y[i] = s.y[i];
-    k.b[i] = s.b[i];
-    k.c[i] = s.c[i];
-    k.d[i] = s.d[i];
+  for (int i = 0; i < s.n && i < 27; ++i) {  // rounded to float: the rates are evaluated in float
+    k.x[i] = (float)s.x[i];
+    k.y[i] = (float)s.y[i];
+    k.b[i] = (float)s.b[i];
+    k.c[i] = (float)s.c[i];
+    k.d[i] = (float)s.d[i];
   }
 }
 
