@@ -625,7 +625,8 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT)) __attribute__((amdgpu_waves_p
   const float* wr = Wt + k * wstride + ((long)aky * NKZB + b) * NH1 * TW + c;
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
   const bool nyq = (b == 0);  // this tile holds the kz=0 slot, which carries the Nyquist plane
-  for (int f = 0; f < nfield; ++f) {
+  {
+    const int f = blockIdx.y;  // one field per workgroup
     {
       // the tile and this radius's window, written to LDS as the loads arrive
       // (measured: keeping the windowed tile in registers for a first
