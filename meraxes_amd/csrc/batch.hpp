@@ -172,7 +172,7 @@ struct BatchPipe {
       const bool win = (w != nullptr);
       if (win) {  // the table layout follows the tile width
         const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
-        k_window_table<N, TW><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, c->stream>>>(c->wtab, *w);
+        k_window_table<N, TW><<<window_table_blocks<N, TW>(), kNT, 0, c->stream>>>(c->wtab, *w);
         MRX_TRY(check_launch(ctx, "k_window_table (batch)"));
       }
       const dim3 gx(N * M::NKZB, c->nbox);

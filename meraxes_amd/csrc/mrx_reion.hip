@@ -528,7 +528,7 @@ struct Pipe {
       MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt));
       {
         PROF("window_table");
-        k_window_table<N, Cfg<N>::TW><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->wtab, w);
+        k_window_table<N, Cfg<N>::TW><<<window_table_blocks<N, Cfg<N>::TW>(), kNT, 0, ctx->stream>>>(ctx->wtab, w);
       }
       MRX_TRY(check_launch(ctx, "k_window_table"));
       Wt = ctx->wtab;
@@ -561,7 +561,7 @@ struct Pipe {
     MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt * K));
     for (int k = 0; k < K; ++k) {
       PROF("window_table");
-      k_window_table<N, Cfg<N>::TW><<<(unsigned)((nt + kNT - 1) / kNT), kNT, 0, ctx->stream>>>(ctx->wtab + k * nt,
+      k_window_table<N, Cfg<N>::TW><<<window_table_blocks<N, Cfg<N>::TW>(), kNT, 0, ctx->stream>>>(ctx->wtab + k * nt,
                                                                                               ws[k]);
     }
     MRX_TRY(check_launch(ctx, "k_window_table"));

@@ -440,25 +440,31 @@ __global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass_sub(float2*
 // blocks of TW): W(a, b, c) for c < N/2 at ((b * NKZB + c / TW) * NH1 + a) *
 // TW + c % TW, so the x-pass tile (|ky| = b, kz block) reads one contiguous
 // NH1 x TW block; the Nyquist column c = N/2 follows at NH1*(N/2)*NH1 + b*NH1 + a.
+//
+// One workgroup per contiguous block of the table: (b, kz block) for c < N/2,
+// NH1 x TW values, or row b of the Nyquist column, NH1 values; the values are
+// stored in table order, so the writes are contiguous.  Every (a, b) is
+// evaluated (the mirror (b, a) gives the same float, as k^2 sums kx^2 and
+// ky^2 first).
+template <int N, int TW>
+constexpr unsigned window_table_blocks()
+{
+  return (unsigned)((N / 2 + 1) * ((N / 2) / TW) + (N / 2 + 1));
+}
+
 template <int N, int TW>
 __global__ __launch_bounds__(kNT) void k_window_table(float* __restrict__ Wt, WinArgs w)
 {
   constexpr int NH = N / 2, NH1 = NH + 1, NKZB = NH / TW;
-  const long idx = (long)blockIdx.x * kNT + threadIdx.x;
-  if (idx >= (long)NH1 * NH1 * NH1) return;
-  const int c = (int)(idx % NH1);
-  const long ab = idx / NH1;
-  const int b = (int)(ab % NH1), a = (int)(ab / NH1);
-  // kx^2 + ky^2 is commutative in float, so W(a, b, c) = W(b, a, c): the
-  // threads with a <= b evaluate the window once for both
-  if (a > b) return;
-  const float v = window(a, b, c, N, w);
-  if (c < NH) {
-    Wt[(((long)b * NKZB + c / TW) * NH1 + a) * TW + c % TW] = v;
-    Wt[(((long)a * NKZB + c / TW) * NH1 + b) * TW + c % TW] = v;
+  const int blk = blockIdx.x;
+  if (blk < NH1 * NKZB) {
+    const int b = blk / NKZB, kb = blk % NKZB;
+    float* out = Wt + (long)blk * NH1 * TW;
+    for (int e = threadIdx.x; e < NH1 * TW; e += kNT) out[e] = window(e / TW, b, kb * TW + e % TW, N, w);
   } else {
-    Wt[(long)NH1 * NH * NH1 + (long)b * NH1 + a] = v;
-    Wt[(long)NH1 * NH * NH1 + (long)a * NH1 + b] = v;
+    const int b = blk - NH1 * NKZB;
+    float* out = Wt + (long)NH1 * NH * NH1 + (long)b * NH1;
+    for (int a = threadIdx.x; a < NH1; a += kNT) out[a] = window(a, b, NH, N, w);
   }
 }
 
