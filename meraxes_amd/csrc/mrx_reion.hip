@@ -427,6 +427,22 @@ template <int N>
 struct Pipe {
   static constexpr int NH = N / 2;
   static constexpr long nc = (long)N * N * NH;
+  // the y passes (tile = (x, kz block), rows along y with stride N/2): at
+  // 512^3 tiles 32 kz wide (256-byte row segments), one 1024-thread
+  // workgroup per CU (793 us per four-field inverse pass against 847 us for
+  // 16-wide tiles and two 512-thread workgroups per CU; the windowed x pass
+  // measured the other way, 2063 against 1607 us per two-radius launch)
+  static constexpr int TWY = (N == 512) ? 32 : Cfg<N>::TW;
+  static constexpr int NTY = (N == 512) ? 1024 : kColNT;
+  // (the forward y pass, one field per launch, keeps the 16-wide tiles: 199
+  // against 205 us at 512^3)
+  template <bool INV>
+  static void colpass_y(mrx_ctx* ctx, float2* data, int nfield, double div)
+  {
+    const dim3 gy(N * (NH / TWY), nfield);
+    k_colpass<N, INV, TWY, 0, NTY><<<gy, TileMap<N, TWY, NTY>::NT, 0, ctx->stream>>>(data, ctx->tw, nc, (long)N * NH,
+                                                                                   NH, div);
+  }
 
   // padded real fields -> unfiltered packed spectra U (scaled by 1/N^3 when
   // `scale`), Nyquist planes split into V.
@@ -544,9 +560,8 @@ struct Pipe {
                                                                       0, 0, w, N, NH / Cfg<N>::TW);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter"));
-    const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
     PROF("colpass_y_inv");
-    k_colpass<N, true><<<gy, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, 0.0);
+    colpass_y<true>(ctx, T, nfield, 0.0);
     return check_launch(ctx, "k_colpass y inv");
   }
 
@@ -557,25 +572,23 @@ struct Pipe {
   static int inverse_xy_k(mrx_ctx* ctx, const float2* U, const float2* V, float2* TK, long tstride, int nfield,
                           const WinArgs* ws, int K)
   {
+    constexpr int TW = Cfg<N>::TW;
     const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
     MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt * K));
     for (int k = 0; k < K; ++k) {
       PROF("window_table");
-      k_window_table<N, Cfg<N>::TW><<<window_table_blocks<N, Cfg<N>::TW>(), kNT, 0, ctx->stream>>>(ctx->wtab + k * nt,
-                                                                                              ws[k]);
+      k_window_table<N, TW><<<window_table_blocks<N, TW>(), kNT, 0, ctx->stream>>>(ctx->wtab + k * nt, ws[k]);
     }
     MRX_TRY(check_launch(ctx, "k_window_table"));
     {
       PROF("ix_filter_k");
-      k_ix_filter_k<N><<<dim3(N * (NH / Cfg<N>::TW) * K, nfield), Cfg<N>::COL_NT, 0, ctx->stream>>>(
+      k_ix_filter_k<N><<<dim3(N * (NH / TW) * K, nfield), Cfg<N>::COL_NT, 0, ctx->stream>>>(
         U, V, TK, ctx->tw, ctx->wtab, nt, nc, (long)N * N, tstride, nfield, K);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter_k"));
-    const dim3 gy(N * (NH / Cfg<N>::TW), nfield);
     for (int k = 0; k < K; ++k) {
       PROF("colpass_y_inv");
-      k_colpass<N, true><<<gy, Cfg<N>::COL_NT, 0, ctx->stream>>>(TK + k * tstride, ctx->tw, nc, (long)N * NH, NH,
-                                                                  0.0);
+      colpass_y<true>(ctx, TK + k * tstride, nfield, 0.0);
     }
     return check_launch(ctx, "k_colpass y inv");
   }
@@ -659,7 +672,7 @@ struct Pipe {
     const dim3 gc(N * (NH / Cfg<N>::TW), nfield);
     k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, NH, (long)N * NH, 0.0);
     MRX_TRY(check_launch(ctx, "k_colpass x inv"));
-    k_colpass<N, true><<<gc, Cfg<N>::COL_NT, 0, ctx->stream>>>(T, ctx->tw, nc, (long)N * NH, NH, 0.0);
+    colpass_y<true>(ctx, T, nfield, 0.0);
     return check_launch(ctx, "k_colpass y inv");
   }
 };

@@ -359,12 +359,12 @@ struct TileMap {
 // One N x TW tile per workgroup: pencils along an axis with element stride
 // `rstride`; tile t covers kz block (t mod NKZB) of line (t / NKZB), line
 // stride astride; blockIdx.y selects the field (stride fstride).
-template <int N, bool INV, int TW = Cfg<N>::TW, int SV = 0>
-__global__ __launch_bounds__((TileMap<N, TW>::NT), 4) void k_colpass(float2* __restrict__ data,
-                                                               const float2* __restrict__ twN, long fstride,
-                                                               long astride, long rstride, double div)
+template <int N, bool INV, int TW = Cfg<N>::TW, int SV = 0, int MAXNT = kColNT>
+__global__ __launch_bounds__((TileMap<N, TW, MAXNT>::NT), 4) void k_colpass(float2* __restrict__ data,
+                                                                   const float2* __restrict__ twN, long fstride,
+                                                                   long astride, long rstride, double div)
 {
-  using M = TileMap<N, TW>;
+  using M = TileMap<N, TW, MAXNT>;
   constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
   __shared__ float2 s[N * SP];
   __shared__ float2 tw[N];
@@ -601,13 +601,13 @@ __global__ __launch_bounds__((TileMap<N, TW, ix_max_threads(N)>::NT)) void k_ix_
 // the whole tile: by then 64 workgroups per XCD had streamed 8 MiB through
 // its 4 MiB L2, and the repeat missed.)  The outputs go out with streaming
 // stores, which keep them from evicting the tiles.
-template <int N, int TW = Cfg<N>::TW>
-__global__ __launch_bounds__((TileMap<N, TW>::NT)) __attribute__((amdgpu_waves_per_eu(4))) void k_ix_filter_k(
+template <int N, int TW = Cfg<N>::TW, int MAXNT = kColNT>
+__global__ __launch_bounds__((TileMap<N, TW, MAXNT>::NT)) __attribute__((amdgpu_waves_per_eu(4))) void k_ix_filter_k(
   const float2* __restrict__ U, const float2* __restrict__ V, float2* __restrict__ T,
   const float2* __restrict__ twN, const float* __restrict__ Wt, long wstride, long fstride, long vstride,
   long tstride, int nfield, int K)
 {
-  using M = TileMap<N, TW>;  // kColNT threads, as k_colpass
+  using M = TileMap<N, TW, MAXNT>;
   constexpr int NT = M::NT, SP = M::SP, RPI = M::RPI, EPT = M::EPT, NKZB = M::NKZB;
   constexpr int NH = N / 2, NH1 = NH + 1;
   constexpr unsigned rs = (unsigned)N * NH;
