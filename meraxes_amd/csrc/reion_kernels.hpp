@@ -1245,7 +1245,8 @@ __global__ __launch_bounds__(256) void k_c2r_z_reg(const float2* __restrict__ in
   float* o = out + blockIdx.y * out_fstride + row * pitch;
 #pragma unroll
   for (int j = 0; j < 16; ++j)
-    *reinterpret_cast<float2*>(o + 2 * (t + A * j)) = make_float2(clamp_kind(z[j].x, clamp), clamp_kind(z[j].y, clamp));
+    store_nt(reinterpret_cast<float2*>(o + 2 * (t + A * j)),
+             make_float2(clamp_kind(z[j].x, clamp), clamp_kind(z[j].y, clamp)));
   if (pitch > N && t == 0) *reinterpret_cast<float2*>(o + N) = make_float2(0.0f, 0.0f);
 }
 
