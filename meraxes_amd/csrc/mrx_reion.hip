@@ -582,7 +582,7 @@ struct Pipe {
     MRX_TRY(check_launch(ctx, "k_window_table"));
     {
       PROF("ix_filter_k");
-      k_ix_filter_k<N><<<dim3(N * (NH / TW) * K, nfield), Cfg<N>::COL_NT, 0, ctx->stream>>>(
+      k_ix_filter_k<N><<<N * (NH / TW) * K * nfield, Cfg<N>::COL_NT, 0, ctx->stream>>>(
         U, V, TK, ctx->tw, ctx->wtab, nt, nc, (long)N * N, tstride, nfield, K);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter_k"));

@@ -616,14 +616,19 @@ __global__ __launch_bounds__((TileMap<N, TW, MAXNT>::NT)) __attribute__((amdgpu_
   constexpr int kTiles = N * NKZB, kXcd = 8;
   const int tid = threadIdx.x;
   const int c = tid % TW, n0 = tid / TW;
-  int tile, k;
+  // one workgroup per (tile, radius, field), grid tiles x K x nfield: the K
+  // nfield workgroups of a tile (the same window block per radius, the same
+  // spectrum tile per field) are dealt to one XCD back to back
+  int tile, k, f;
   if constexpr (kTiles % kXcd == 0) {
-    const int slot = blockIdx.x / kXcd;  // < kTiles K / 8, so tile < kTiles
-    k = slot % K;
-    tile = (slot / K) * kXcd + blockIdx.x % kXcd;
+    const int slot = blockIdx.x / kXcd;  // < kTiles K nfield / 8, so tile < kTiles
+    f = slot % nfield;
+    k = (slot / nfield) % K;
+    tile = (slot / (nfield * K)) * kXcd + blockIdx.x % kXcd;
   } else {
-    k = blockIdx.x % K;
-    tile = blockIdx.x / K;
+    f = blockIdx.x % nfield;
+    k = (blockIdx.x / nfield) % K;
+    tile = blockIdx.x / (nfield * K);
   }
   const int ky = tile / NKZB, b = tile % NKZB;
   const int aky = ky <= NH ? ky : N - ky;
@@ -632,7 +637,6 @@ __global__ __launch_bounds__((TileMap<N, TW, MAXNT>::NT)) __attribute__((amdgpu_
   for (int i = tid; i < N; i += NT) tw[i] = twN[i];
   const bool nyq = (b == 0);  // this tile holds the kz=0 slot, which carries the Nyquist plane
   {
-    const int f = blockIdx.y;  // one field per workgroup
     {
       // the tile and this radius's window, written to LDS as the loads arrive
       // (measured: keeping the windowed tile in registers for a first
