@@ -591,16 +591,20 @@ __global__ __launch_bounds__((TileMap<N, TW, ix_max_threads(N)>::NT)) void k_ix_
 // k_window_table's layout), transforms it and writes it to T + k tstride.  Per
 // radius the result is k_ix_filter<N, kWinTable>'s, operation for operation.
 //
-// One workgroup per (tile, radius), grid K x (tiles).  The K workgroups of a
-// tile read the same U tile, so they are made to run side by side on one XCD,
-// whose L2 then serves every read but the first: workgroups are dealt to the
-// 8 XCDs round robin (workgroup g to XCD g mod 8), so with the tiles a
-// multiple of 8, g = (j K + k) 8 + x gives tile 8 j + x, radius k, and the K
-// radii of a tile are dealt consecutively to XCD x.  (A workgroup that looped
-// over the K radii itself re-read its tile after a transform and a store of
-// the whole tile: by then 64 workgroups per XCD had streamed 8 MiB through
-// its 4 MiB L2, and the repeat missed.)  The outputs go out with streaming
-// stores, which keep them from evicting the tiles.
+// One workgroup per (tile, radius, field), grid tiles x K x nfield.  The K
+// workgroups of a (tile, field) read the same U tile and the nfield of a
+// (tile, radius) the same window block, so all K nfield of a tile are made to
+// run side by side on one XCD, whose L2 then serves every read but the first:
+// workgroups are dealt to the 8 XCDs round robin (workgroup g to XCD g mod
+// 8), so with the tiles a multiple of 8, g = ((j K + k) nfield + f) 8 + x
+// gives tile 8 j + x, radius k, field f, and a tile's workgroups are dealt
+// consecutively to XCD x.  (A workgroup that looped over the K radii itself
+// re-read its tile after a transform and a store of the whole tile: by then
+// 64 workgroups per XCD had streamed 8 MiB through its 4 MiB L2, and the
+// repeat missed.  One that looped over the fields took 1650 us per
+// two-radius launch at 512^3 against 1603 us for one field per workgroup.)
+// The outputs go out with streaming stores, which keep them from evicting the
+// tiles.
 template <int N, int TW = Cfg<N>::TW, int MAXNT = kColNT>
 __global__ __launch_bounds__((TileMap<N, TW, MAXNT>::NT)) __attribute__((amdgpu_waves_per_eu(4))) void k_ix_filter_k(
   const float2* __restrict__ U, const float2* __restrict__ V, float2* __restrict__ T,
