@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mrx_reion.h"
@@ -1129,6 +1130,24 @@ struct TsRun {
         // input grids only: taken first, so the host builds the tables while
         // the device filters the shells
         if (!p->fixed_globals) MRX_TRY(fcoll(s->R[0], 1));
+        if (p->fixed_globals) {
+          collapse_fraction = p->fixed_collapse_fraction;
+          x_e_ave = p->fixed_x_e_ave;
+        }
+        s->collapse_fraction = collapse_fraction;
+        s->x_e_ave = x_e_ave;
+        // the host tables are built on a thread of their own while this one
+        // queues the shells' kernels (the build reads the plan's radii and
+        // edges and writes only the tables; the shell loop reads the plan)
+        int tab_rc = MRX_OK;
+        std::thread tab([&tab_rc, s, d, p] { tab_rc = mrx_ts_build_tables(s, d, p); });
+        struct Join {
+          std::thread& t;
+          ~Join()
+          {
+            if (t.joinable()) t.join();
+          }
+        } join_tab{tab};
         const long half = (long)N * N * (N / 2 + 1);
         // the working grid starts as the input sfr grid: it is read in place
         // until the first shell with operations writes ts_work from it
@@ -1171,14 +1190,8 @@ struct TsRun {
         }
         ctx->ts_shells_N = N;
         ctx->ts_shells_n = s->nsteps;
-        if (p->fixed_globals) {
-          collapse_fraction = p->fixed_collapse_fraction;
-          x_e_ave = p->fixed_x_e_ave;
-        }
-        s->collapse_fraction = collapse_fraction;
-        s->x_e_ave = x_e_ave;
-        const int rc = mrx_ts_build_tables(s, d, p);
-        if (rc != MRX_OK) return set_err(ctx, rc, "spin-temperature tables failed");
+        tab.join();
+        if (tab_rc != MRX_OK) return set_err(ctx, tab_rc, "spin-temperature tables failed");
         fill_ts_dev(*ctx->ts_host, d, s, p);
         HIP_TRY(hipMemcpyAsync(ctx->ts_dev, ctx->ts_host, sizeof(TsDev), hipMemcpyHostToDevice, ctx->stream));
         if (prepare_only) return MRX_OK;
