@@ -1451,10 +1451,12 @@ __global__ __launch_bounds__(kNT) void k_brightness_temp(const float* __restrict
 #pragma clang fp contract(off)
   __shared__ double red[kNT];
   double acc = 0.0;
-  const long stride = (long)gridDim.x * kNT;
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < ncell; i += stride) {
-    const long r = i / a.N;
-    const int z = (int)(i % a.N);
+  // rows of the grid over the workgroups, cells of a row over the threads
+  // (no per-cell index division); ncell = N^3
+  const long nrow = ncell / a.N;
+  for (long r = blockIdx.x; r < nrow; r += gridDim.x)
+  for (int z = threadIdx.x; z < a.N; z += kNT) {
+    const long i = r * a.N + z;
     const double pixel_deltax = deltax[r * (a.N + 2) + z];
     const double pixel_x_HI = xH[i];
     float v = (float)(a.const_factor * pixel_x_HI * (1.0 + pixel_deltax));

@@ -488,12 +488,12 @@ __global__ __launch_bounds__(kNT) void k_ts_fcoll(const float* __restrict__ star
                                                  FcollArgs a)
 {
   __shared__ double red[kNT];
-  const long ncell = (long)a.N * a.N * a.N;
   double sf = 0.0, sx = 0.0;
-  const long stride = (long)gridDim.x * kNT;
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < ncell; i += stride) {
-    const long r = i / a.N;
-    const int z = (int)(i % a.N);
+  // rows of the grid over the workgroups, cells of a row over the threads
+  // (no per-cell index division)
+  const long nrow = (long)a.N * a.N;
+  for (long r = blockIdx.x; r < nrow; r += gridDim.x)
+  for (int z = threadIdx.x; z < a.N; z += kNT) {
     const long ip = r * (a.N + 2) + z;
     const double dom = 1.0 + deltax[ip];
     double c = stars[ip] / (a.M_h2 * dom) * (4.0 / 3.0) * M_PI * a.R3 / a.pixel_volume;
