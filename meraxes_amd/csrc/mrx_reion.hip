@@ -1140,7 +1140,12 @@ struct TsRun {
         // queues the shells' kernels (the build reads the plan's radii and
         // edges and writes only the tables; the shell loop reads the plan)
         int tab_rc = MRX_OK;
-        std::thread tab([&tab_rc, s, d, p] { tab_rc = mrx_ts_build_tables(s, d, p); });
+        std::thread tab;
+        try {
+          tab = std::thread([&tab_rc, s, d, p] { tab_rc = mrx_ts_build_tables(s, d, p); });
+        } catch (...) {  // no thread to be had: build them here, before the shells
+          tab_rc = mrx_ts_build_tables(s, d, p);
+        }
         struct Join {
           std::thread& t;
           ~Join()
@@ -1190,7 +1195,7 @@ struct TsRun {
         }
         ctx->ts_shells_N = N;
         ctx->ts_shells_n = s->nsteps;
-        tab.join();
+        if (tab.joinable()) tab.join();
         if (tab_rc != MRX_OK) return set_err(ctx, tab_rc, "spin-temperature tables failed");
         fill_ts_dev(*ctx->ts_host, d, s, p);
         HIP_TRY(hipMemcpyAsync(ctx->ts_dev, ctx->ts_host, sizeof(TsDev), hipMemcpyHostToDevice, ctx->stream));
