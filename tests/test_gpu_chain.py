@@ -88,3 +88,58 @@ def test_thirty_snapshot_chain_64(ctx):
               f"(|d| {dxh:.1e}, flips {int(flips.sum())}) | <dTb> gpu {ag:.5f} oracle {ao:.5f}", flush=True)
     # the run reionises part of the box and heats the IGM above the CMB by z = 6
     assert og["volume_weighted_global_xH"] < 0.99 and tg["volume_ave_TK"] > 2.728 * 7
+
+
+def test_snapshot_chain_256(ctx):
+    """One snapshot of the full-snapshot configuration at 256^3 on a 200 Mpc/h
+    box (the 512^3, 400 Mpc/h cell size, so the same 40 shell radii and the same
+    radius ladder shape, 44 radii): snapshot 24 (z = 8.41) with an 8-slot SFR
+    history, x_e = 2e-3 and T_K = 30 K before the step - Ts, the bubble finder
+    with x_e as the fourth field and the UVB, delta_T with T_S - against the
+    oracle, at the tolerances of the 64^3 chain."""
+    N, L, eff, snap = 256, 200.0, 20.0, 24
+    run = SyntheticRun(N, L, seed=9)
+    z = float(run.z[snap])
+    d, s, f = run.inputs(snap)
+    nh = 8
+    hist = np.stack([f * np.float32(((1 + z) / (1 + run.z[max(snap - k, 0)])) ** 3) for k in range(nh)])
+    fc = {k: np.zeros(len(run.z)) for k in "go"}
+    for k in "go":
+        fc[k][:snap] = np.linspace(1e-4, 5e-3, snap)
+    PR = 2 * (N // 2 + 1)
+    xe = np.full((N, N, PR), 2e-3, np.float32)
+    Tk = np.full((N, N, N), 30.0, np.float32)
+    print(f"{N}^3 chain snapshot {snap} z={z:.3f}: spin temperature", flush=True)
+    tg = reion.compute_ts(ctx, P.ts_params(N, L, snap, run.z, fc["g"], nh), d, s, f, hist, xe.copy(), Tk.copy())
+    to = O.compute_ts(P.ts_params(N, L, snap, run.z, fc["o"], nh), d, s, f, hist, xe.copy(), Tk.copy())
+    assert tg["homogeneous"] == to["homogeneous"] == 0 and tg["no_light"] == to["no_light"] == 0
+    rx = rel(tg["x_e_box"][:, :, :N], to["x_e_box"][:, :, :N]).max()
+    rT = rel(tg["Tk_box"], to["Tk_box"]).max()
+    rS = rel(tg["TS_box"], to["TS_box"]).max()
+    print(f"  x_e {rx:.1e} T_K {rT:.1e} T_S {rS:.1e} | <T_S> gpu {tg['volume_ave_TS']:.5f} "
+          f"oracle {to['volume_ave_TS']:.5f}", flush=True)
+    assert rx <= 1e-4 and rT <= 1e-4 and rS <= 1e-4, (rx, rT, rS)
+    for k in ("volume_ave_TS", "volume_ave_TK", "volume_ave_xe", "volume_ave_J_alpha", "volume_ave_Xheat"):
+        assert abs(tg[k] - to[k]) <= 1e-5 * abs(to[k]), k
+    print(f"{N}^3 chain: bubble finder", flush=True)
+    hp = P.hii_params(N, L, z, efficiency=eff, uvb_flag=1, include_spin_temp=1)
+    og = reion.find_hii_bubbles(ctx, hp, d, s, f, x_e=tg["x_e_box"])
+    oo = O.find_hii_bubbles(hp, d, s, f, x_e=to["x_e_box"], want_ratio=True)
+    assert og["n_radii"] == oo["n_radii"] == 44
+    flips = (og["xH"] < 1e-5) != (oo["xH"] < 1e-5)
+    margin = np.abs(oo["max_ratio"].astype(np.float64) - 1.0)
+    dxh = abs(og["volume_weighted_global_xH"] - oo["volume_weighted_global_xH"])
+    print(f"  <xH> gpu {og['volume_weighted_global_xH']:.7f} oracle {oo['volume_weighted_global_xH']:.7f} "
+          f"flips {int(flips.sum())}", flush=True)
+    assert dxh <= 1e-5 and flips.mean() <= 1e-4
+    if flips.any():
+        assert margin[flips].max() <= 1e-4
+    bp = P.tb_params(N, z, include_spin_temp=1)
+    dg, ag = reion.brightness_temp(ctx, bp, og["xH"], d, tg["TS_box"])
+    do, ao = O.brightness_temp(bp, oo["xH"], d, to["TS_box"])
+    keep = ~flips
+    rms = float(np.sqrt(np.mean(do.astype(np.float64) ** 2)))
+    dd = np.abs(dg[keep].astype(np.float64) - do[keep])
+    print(f"  <dTb> gpu {ag:.5f} oracle {ao:.5f}, max |d| {dd.max():.2e} (rms {rms:.3f})", flush=True)
+    assert (dd <= 1e-4 * np.abs(do[keep]) + 1e-4 * rms).all()
+    assert abs(ag - ao) <= 1e-4 * abs(ao) + 1e-6 * rms
