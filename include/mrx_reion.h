@@ -363,8 +363,12 @@ int mrx_power_spectrum_adjoint(mrx_ctx* ctx, int grid_dim, double box_size_mpc, 
  * nullable).  The ionisation mask is a stop-gradient (0 almost everywhere);
  * cells partly ionised on the unfiltered last step carry the derivative of
  * xH = n_f - eff f_coll.  `ts` NULL: no spin temperature (tb and hii must not
- * use it).  With ts, the shell tables' two grid means are held at their
- * forward values (set ts->fixed_globals for an exactly matching forward).
+ * use it).  With ts, the gradient reaches through the shell tables' two grid
+ * means (the collapse fraction and <x_e>, ComputeTs.c:413-475): their
+ * derivatives are taken by central differences of the host table build and
+ * spread back over the cells the means are taken from.  ts->fixed_globals
+ * holds them at ts->fixed_collapse_fraction and ts->fixed_x_e_ave instead,
+ * forward and adjoint alike (a stop-gradient on the means).
  * Per-radius filtered fields are recomputed, not stored.  Device pointers only.
  */
 int mrx_snapshot_adjoint(mrx_ctx* ctx, const mrx_ts_params* ts, const mrx_hii_params* hii, const mrx_tb_params* tb,

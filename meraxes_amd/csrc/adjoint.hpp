@@ -264,6 +264,72 @@ __global__ __launch_bounds__(kNT) void k_ts_shell_grad(const TsDev* __restrict__
   if (z == N - 1) out[ip + 1] = out[ip + 2] = 0.0f;
 }
 
+// The shell tables' dependence on the two grid means they are built from
+// (the collapse fraction, m = 0, and <x_e>, m = 1; ComputeTs.c:413-475, which
+// enter through tau_X): per cell, sum over shells of v_ct times
+// sum_j gS_j (E[m][lo][ct][j] + fr E[m][lo][ct][j + 3]), j = heat, ion, Ly-alpha,
+// where E is the derivative of the six x_e tables with respect to mean m
+// times the shell's weight (sfr_factor dt_dzpp dzpp zpp_pow), from the host;
+// block sums of the nm values to partials[block * 2 + m].
+__global__ __launch_bounds__(kNT) void k_ts_means_grad(const TsDev* __restrict__ tg, const float* __restrict__ shells,
+                                                      const float* __restrict__ xe_prev,
+                                                      const double* __restrict__ gS, const double* __restrict__ E,
+                                                      int nm, int N, double* __restrict__ partials)
+{
+  __shared__ double red[2 * kNT / 64];
+  const TsDev& t = *tg;
+  const int ns = t.nsteps;
+  const long ncell = (long)N * N * N;
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < ncell) {
+    const long r = i / N;
+    const int z = (int)(i % N);
+    const XeBracket br = xe_bracket(t, xe_prev[r * (N + 2) + z]);
+    const double g0 = gS[i], g1 = gS[ncell + i], g2 = gS[2 * ncell + i];
+    for (int ct = 0; ct < ns; ++ct) {
+      const double v = (double)shells[(long)ct * ncell + i];
+      if (v == 0.0) continue;
+      for (int m = 0; m < nm; ++m) {
+        const double* e = E + (((long)m * kNXHII + br.lo) * ns + ct) * 6;
+        acc[m] += v * (g0 * (e[0] + br.fr * e[3]) + g1 * (e[1] + br.fr * e[4]) + g2 * (e[2] + br.fr * e[5]));
+      }
+    }
+  }
+  block_sum_n<kNT, 2>(acc, red);
+  if (threadIdx.x == 0) {
+    partials[(long)blockIdx.x * 2] = acc[0];
+    partials[(long)blockIdx.x * 2 + 1] = acc[1];
+  }
+}
+
+// The means' gradients back to the cells: <x_e> is the mean of x_e_prev, so
+// every cell's x_e_prev gradient gains gxe (dL/d<x_e> / N^3); the collapse
+// fraction is the mean of min(c, 1), c = stars / (RtoM(R) h^2 (1 + delta))
+// (4/3) pi R^3 / V (k_ts_fcoll), so the cells with c < 1 gain gcf c / stars
+// on stars and -gcf c / (1 + delta) on delta (gcf = dL/d(collapse fraction) /
+// N^3).
+__global__ __launch_bounds__(kNT) void k_ts_means_apply(const float* __restrict__ deltax,
+                                                       const float* __restrict__ stars, float* __restrict__ g_delta,
+                                                       float* __restrict__ g_stars, float* __restrict__ g_xe_prev,
+                                                       double gxe, double gcf, FcollArgs a)
+{
+  const long ncell = (long)a.N * a.N * a.N;
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= ncell) return;
+  const long ip = (i / a.N) * (a.N + 2) + i % a.N;
+  if (g_xe_prev) g_xe_prev[ip] += (float)gxe;
+  if (gcf != 0.0) {
+    const double dom = 1.0 + deltax[ip];
+    const double k = 1.0 / (a.M_h2 * dom) * (4.0 / 3.0) * M_PI * a.R3 / a.pixel_volume;
+    const double c = (double)stars[ip] * k;
+    if (c < 1.0) {
+      if (g_stars) g_stars[ip] += (float)(gcf * k);
+      if (g_delta) g_delta[ip] -= (float)(gcf * c / dom);
+    }
+  }
+}
+
 // acc[i] += x[i] over n floats
 __global__ void k_axpy(float* __restrict__ acc, const float* __restrict__ x, long n)
 {

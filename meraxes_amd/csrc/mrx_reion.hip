@@ -134,6 +134,8 @@ struct mrx_ctx {
   size_t adj_work_cap = 0;
   double* adj_gS = nullptr;
   size_t adj_gS_cap = 0;
+  double* adj_E = nullptr;  // the shell tables' derivatives with respect to their grid means
+  size_t adj_E_cap = 0;
   // recombination-rate tables (host, built once per No and Y_He) and their device copy
   mrx_rr_tables* rr = nullptr;
   RRDev* rr_dev = nullptr;
@@ -1458,6 +1460,79 @@ struct PowerSpec {
 struct SnapAdj {
   template <int N>
   struct At {
+    // The gradient through the shell tables' two grid means, the collapse
+    // fraction and <x_e> (ComputeTs.c:413-475), which the tables take through
+    // tau_X: the derivatives of the six x_e tables with respect to each mean
+    // by central differences of the host table build (relative step 1e-4),
+    // contracted on the device with every cell's shell values and shell-sum
+    // adjoints (k_ts_means_grad), then spread back over the cells the means
+    // are taken from (k_ts_means_apply).  <x_e> only when its gradient
+    // (g_xe_prev) is wanted.
+    static int means_term(mrx_ctx* ctx, const mrx_ts_params* tp, const mrx_ts_snap* s, const float* deltax,
+                          const float* stars, const float* xe_prev, float* g_delta, float* g_stars, float* g_xe_prev)
+    {
+      const int ns = s->nsteps;
+      const int nm = g_xe_prev ? 2 : 1;
+      const long ncell = (long)N * N * N;
+      std::vector<double> E((size_t)nm * kNXHII * ns * 6, 0.0);
+      std::unique_ptr<mrx_ts_snap> sc(new mrx_ts_snap);
+      std::unique_ptr<TsDev> tplus(new TsDev), tminus(new TsDev);
+      const TsDev& t0 = *ctx->ts_host;  // the forward's constants
+      // the table build records the collapse fraction in the caller's history
+      double* fc_slot = tp->stored_fcoll ? tp->stored_fcoll + tp->snapshot : nullptr;
+      const double fc_saved = fc_slot ? *fc_slot : 0.0;
+      int rc = MRX_OK;
+      for (int m = 0; m < nm && rc == MRX_OK; ++m) {
+        const double x0 = (m == 0) ? s->collapse_fraction : s->x_e_ave;
+        const double h = 1e-4 * std::max(std::fabs(x0), 1e-8);
+        for (int sg = 0; sg < 2 && rc == MRX_OK; ++sg) {
+          *sc = *s;
+          (m == 0 ? sc->collapse_fraction : sc->x_e_ave) = x0 + (sg ? -h : h);
+          rc = mrx_ts_build_tables(sc.get(), ctx->tsd, tp);
+          if (rc == MRX_OK) fill_ts_dev(sg ? *tminus : *tplus, ctx->tsd, sc.get(), tp);
+        }
+        if (rc != MRX_OK) break;
+        for (int lo = 0; lo < kNXHII; ++lo)
+          for (int ct = 0; ct < ns; ++ct) {
+            const double w = t0.sfr_factor * t0.dt_dzpp[ct] * t0.dzpp[ct] * t0.zpp_pow[ct];
+            const double d[6] = {tplus->fheat[lo][ct] - tminus->fheat[lo][ct], tplus->fion[lo][ct] - tminus->fion[lo][ct],
+                                 tplus->flya[lo][ct] - tminus->flya[lo][ct], tplus->sheat[lo][ct] - tminus->sheat[lo][ct],
+                                 tplus->sion[lo][ct] - tminus->sion[lo][ct], tplus->slya[lo][ct] - tminus->slya[lo][ct]};
+            for (int j = 0; j < 6; ++j) E[(((size_t)m * kNXHII + lo) * ns + ct) * 6 + j] = w * d[j] / (2.0 * h);
+          }
+      }
+      if (fc_slot) *fc_slot = fc_saved;
+      if (rc != MRX_OK) return set_err(ctx, rc, "spin-temperature tables (grid-mean derivatives) failed");
+      MRX_TRY(ensure_dev(ctx, ctx->adj_E, ctx->adj_E_cap, E.size()));
+      HIP_TRY(hipMemcpyAsync(ctx->adj_E, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice, ctx->stream));
+      const unsigned nbc = (unsigned)((ncell + kNT - 1) / kNT);
+      {
+        PROF("adjoint_means");
+        k_ts_means_grad<<<nbc, kNT, 0, ctx->stream>>>(ctx->ts_dev, ctx->ts_shells, xe_prev, ctx->adj_gS, ctx->adj_E, nm,
+                                                      N, ctx->partials);
+      }
+      MRX_TRY(check_launch(ctx, "k_ts_means_grad"));
+      MRX_TRY(reduce_partials(ctx, ctx->stream, ctx->partials, (int)nbc, 2, 1, ctx->partials + (size_t)nbc * 2 + 16,
+                              ctx->sums));
+      double g[2];
+      HIP_TRY(hipMemcpyAsync(g, ctx->sums, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      // both means are over the N^3 cells
+      const double n3 = std::pow((double)N, 3);
+      const double h = tp->hubble_h;
+      const double R0 = s->R[0];
+      FcollArgs fa;
+      fa.M_h2 = ts_RtoM(tp, R0) * h * h;
+      fa.R3 = std::pow(R0, 3.0);
+      fa.R3_over_V = 0.0;
+      fa.pixel_volume = std::pow(tp->box_size / h / (double)N, 3);
+      fa.N = N;
+      fa.with_xe = 1;
+      k_ts_means_apply<<<nbc, kNT, 0, ctx->stream>>>(deltax, stars, g_delta, g_stars, nm > 1 ? g_xe_prev : nullptr,
+                                                     nm > 1 ? g[1] / n3 : 0.0, g[0] / n3, fa);
+      return check_launch(ctx, "k_ts_means_apply");
+    }
+
     static int run(mrx_ctx* ctx, const mrx_ts_params* tp, const mrx_hii_params* hp, const mrx_tb_params* bp,
                    const float* deltax, const float* stars, const float* sfr, const float* hist,
                    const float* xe_prev, const float* Tk, const float* xH, const float* TS, const float* xe_box,
@@ -1505,6 +1580,7 @@ struct SnapAdj {
         if (g_xe_prev) HIP_TRY(hipMemsetAsync(g_xe_prev, 0, sizeof(float) * np, ctx->stream));
         if (g_Tk) HIP_TRY(hipMemsetAsync(g_Tk, 0, sizeof(float) * ncell, ctx->stream));
       }
+      if (shells && !tp->fixed_globals) MRX_TRY(means_term(ctx, tp, s, deltax, stars, xe_prev, g_delta, g_stars, g_xe_prev));
       const int nslot = (tp && s && !s->homogeneous) ? tp->n_history : 0;
       if (g_sfr) HIP_TRY(hipMemsetAsync(g_sfr, 0, sizeof(float) * np, ctx->stream));
       if (g_hist && tp) HIP_TRY(hipMemsetAsync(g_hist, 0, sizeof(float) * np * (size_t)tp->n_history, ctx->stream));
@@ -1725,6 +1801,7 @@ int mrx_free(mrx_ctx* ctx)
   if (ctx->TK) (void)hipFree(ctx->TK);
   dfree(ctx->adj_work);
   dfree(ctx->adj_gS);
+  dfree(ctx->adj_E);
   dfree(ctx->tbv);
   dfree(ctx->rr_dev);
   delete ctx->rr;

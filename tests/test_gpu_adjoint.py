@@ -407,12 +407,13 @@ def test_snapshot_adjoint_fd_directions_256(ctx):
 
 
 def test_fixed_table_means_term_64(ctx):
-    """Size of what holding the shell tables' two grid means fixed drops from the
-    gradient (collapse fraction and <x_e>, ComputeTs.c:413-475, which the
-    tables depend on through tau_X): central differences of the loss with the
-    means fixed and with the forward measuring them again, along 32 deltax
-    directions at 64^3.  The difference is reported; it is bounded here at
-    5 %, its measured size being about 3 %."""
+    """The shell tables' two grid means (collapse fraction and <x_e>,
+    ComputeTs.c:413-475, which the tables depend on through tau_X), along 32
+    deltax directions at 64^3: central differences of the loss with the means
+    held fixed and with the forward measuring them again differ by about 3 %
+    (bounded here at 5 %); the adjoint with the means fixed matches the first,
+    and the adjoint with them measured (its gradient through the tables'
+    derivatives with respect to the means) matches the second."""
     N, L = 64, 50.0
     box = L / P.HUBBLE_H
     c = _ts_case(N, L, 31)
@@ -448,7 +449,65 @@ def test_fixed_table_means_term_64(ctx):
     fd_free = _fd_directions(ctx, c, Pref, box, kf, "d", c["d"], dirs, 1e-3)
     ad = np.array([float((gd * v).sum()) for v in dirs])
     dropped = rel_l2(fd_fixed, fd_free)
-    print(f"fixed table means: FD fixed vs measured rel L2 {dropped:.2e}; AD vs FD fixed {rel_l2(ad, fd_fixed):.2e}, "
-          f"AD vs FD measured {rel_l2(ad, fd_free):.2e}", flush=True)
+    _, gd_free, _, _ = _ts_loss(ctx, c, c["d"], c["s"], Pref, box, kf, grad=True)
+    ad_free = np.array([float((gd_free * v).sum()) for v in dirs])
+    print(f"fixed table means: FD fixed vs measured rel L2 {dropped:.2e}; AD (fixed) vs FD fixed "
+          f"{rel_l2(ad, fd_fixed):.2e}, vs FD measured {rel_l2(ad, fd_free):.2e}; AD (measured) vs FD measured "
+          f"{rel_l2(ad_free, fd_free):.2e}", flush=True)
     assert rel_l2(ad, fd_fixed) < 2e-2
     assert dropped < 5e-2
+    assert rel_l2(ad_free, fd_free) < 2e-2
+
+
+def test_table_means_gradient_xe_64(ctx):
+    """<x_e>, one of the two grid means the shell tables are built from, is the
+    mean of x_e_prev: along the uniform direction in x_e_prev (every cell moved
+    by the same step, so <x_e> moves by it too) the adjoint with the means
+    measured matches central differences of the forward that measures them
+    again, and the adjoint with them held fixed matches the forward that holds
+    them fixed; the two derivatives differ by the tables' share, about 1 %
+    here, which the adjoint reproduces (64.5 against 61.7 measured)."""
+    N, L = 64, 50.0
+    box = L / P.HUBBLE_H
+    c = _ts_case(N, L, 41)
+    g0 = reion.compute_ts(ctx, c["tsp"], c["d"], c["s"], c["w"], c["hist"], c["xe0"].clone(), c["Tk0"].clone())
+    free = c["tsp"]
+    fixed = P.ts_params(N, L, 24, c["zz"], c["fc"].copy(), 8, on_device=1)
+    fixed.fixed_globals = 1
+    fixed.fixed_collapse_fraction = g0["collapse_fraction"]
+    fixed.fixed_x_e_ave = g0["x_e_ave"]
+    c2 = _ts_case(N, L, 42)
+    kf = MA.k_factor_for_bins(N, 12)
+    dT2, _ = MA.snapshot(ctx, fixed, c2["hp"], c2["bp"], c2["d"], c2["s"], c2["w"], c2["w"], c2["hist"],
+                         c2["xe0"], c2["Tk0"])
+    Pref = MA.power_spectrum(ctx, dT2.detach(), box, kf).detach()
+    real = torch.zeros_like(c["xe0"])
+    real[:, :, :N] = 1.0
+
+    def loss(tsp, xe, grad=False):
+        if grad:
+            xe = xe.clone().requires_grad_(True)
+        dT, _ = MA.snapshot(ctx, tsp, c["hp"], c["bp"], c["d"], c["s"], c["w"], c["w"], c["hist"], xe, c["Tk0"])
+        L_ = ((MA.power_spectrum(ctx, dT, box, kf) - Pref) ** 2).sum()
+        if grad:
+            L_.backward()
+            return float((xe.grad * real).sum())
+        return float(L_)
+
+    eps = 2e-6  # 1e-3 of x_e_prev = 2e-3
+    out = {}
+    for name, tsp in (("fixed", fixed), ("measured", free)):
+        fd = (loss(tsp, c["xe0"] + eps * real) - loss(tsp, c["xe0"] - eps * real)) / (2 * eps)
+        ad = loss(tsp, c["xe0"], grad=True)
+        out[name] = (ad, fd)
+    print(f"uniform x_e_prev direction: means fixed AD {out['fixed'][0]:.6e} FD {out['fixed'][1]:.6e}; "
+          f"measured AD {out['measured'][0]:.6e} FD {out['measured'][1]:.6e}", flush=True)
+    for ad, fd in out.values():
+        assert abs(ad - fd) <= 2e-2 * abs(fd)
+    # the tables' share (about 1 % of the derivative at this snapshot): the
+    # adjoint's matches the finite differences' to 20 % of itself
+    share_fd = out["measured"][1] - out["fixed"][1]
+    share_ad = out["measured"][0] - out["fixed"][0]
+    print(f"  the tables' share: AD {share_ad:.3e} FD {share_fd:.3e}", flush=True)
+    assert abs(share_fd) > 2e-3 * abs(out["fixed"][1])
+    assert abs(share_ad - share_fd) <= 0.2 * abs(share_fd)
