@@ -1468,13 +1468,11 @@ struct SnapAdj {
     // adjoints (k_ts_means_grad), then spread back over the cells the means
     // are taken from (k_ts_means_apply).  <x_e> only when its gradient
     // (g_xe_prev) is wanted.
-    static int means_term(mrx_ctx* ctx, const mrx_ts_params* tp, const mrx_ts_snap* s, const float* deltax,
-                          const float* stars, const float* xe_prev, float* g_delta, float* g_stars, float* g_xe_prev)
+    static int means_tables(mrx_ctx* ctx, const mrx_ts_params* tp, const mrx_ts_snap* s, int nm,
+                            std::vector<double>& E)
     {
       const int ns = s->nsteps;
-      const int nm = g_xe_prev ? 2 : 1;
-      const long ncell = (long)N * N * N;
-      std::vector<double> E((size_t)nm * kNXHII * ns * 6, 0.0);
+      E.assign((size_t)nm * kNXHII * ns * 6, 0.0);
       std::unique_ptr<mrx_ts_snap> sc(new mrx_ts_snap);
       std::unique_ptr<TsDev> tplus(new TsDev), tminus(new TsDev);
       const TsDev& t0 = *ctx->ts_host;  // the forward's constants
@@ -1502,7 +1500,13 @@ struct SnapAdj {
           }
       }
       if (fc_slot) *fc_slot = fc_saved;
-      if (rc != MRX_OK) return set_err(ctx, rc, "spin-temperature tables (grid-mean derivatives) failed");
+      return rc;
+    }
+    static int means_term(mrx_ctx* ctx, const mrx_ts_params* tp, const mrx_ts_snap* s, int nm,
+                          const std::vector<double>& E, const float* deltax, const float* stars, const float* xe_prev,
+                          float* g_delta, float* g_stars, float* g_xe_prev)
+    {
+      const long ncell = (long)N * N * N;
       MRX_TRY(ensure_dev(ctx, ctx->adj_E, ctx->adj_E_cap, E.size()));
       HIP_TRY(hipMemcpyAsync(ctx->adj_E, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice, ctx->stream));
       const unsigned nbc = (unsigned)((ncell + kNT - 1) / kNT);
@@ -1568,6 +1572,34 @@ struct SnapAdj {
       }
       const bool shells = (a.ts_mode == 2) && !s->no_light;
       if (a.ts_mode == 2) MRX_TRY(ensure_dev(ctx, ctx->adj_gS, ctx->adj_gS_cap, 4 * (size_t)ncell));
+      // the tables' derivatives with respect to their grid means are host
+      // work: built on a thread of their own while the device runs the
+      // adjoint, and used after the shells' adjoints
+      const bool means = shells && !tp->fixed_globals;
+      const int nm = g_xe_prev ? 2 : 1;
+      std::vector<double> E;
+      int E_rc = MRX_OK;
+      std::thread E_thread;
+      if (means) {
+        try {
+          E_thread = std::thread([&E_rc, &E, ctx, tp, s, nm] { E_rc = means_tables(ctx, tp, s, nm, E); });
+        } catch (...) {
+          E_rc = means_tables(ctx, tp, s, nm, E);
+        }
+      }
+      struct JoinE {
+        std::thread& t;
+        ~JoinE()
+        {
+          if (t.joinable()) t.join();
+        }
+      } join_E{E_thread};
+      auto finish_means = [&]() -> int {
+        if (!means) return MRX_OK;
+        if (E_thread.joinable()) E_thread.join();
+        if (E_rc != MRX_OK) return set_err(ctx, E_rc, "spin-temperature tables (grid-mean derivatives) failed");
+        return means_term(ctx, tp, s, nm, E, deltax, stars, xe_prev, g_delta, g_stars, g_xe_prev);
+      };
       const unsigned nbc = (unsigned)((ncell + kNT - 1) / kNT);
       {
         PROF("adjoint_cell");
@@ -1580,11 +1612,10 @@ struct SnapAdj {
         if (g_xe_prev) HIP_TRY(hipMemsetAsync(g_xe_prev, 0, sizeof(float) * np, ctx->stream));
         if (g_Tk) HIP_TRY(hipMemsetAsync(g_Tk, 0, sizeof(float) * ncell, ctx->stream));
       }
-      if (shells && !tp->fixed_globals) MRX_TRY(means_term(ctx, tp, s, deltax, stars, xe_prev, g_delta, g_stars, g_xe_prev));
       const int nslot = (tp && s && !s->homogeneous) ? tp->n_history : 0;
       if (g_sfr) HIP_TRY(hipMemsetAsync(g_sfr, 0, sizeof(float) * np, ctx->stream));
       if (g_hist && tp) HIP_TRY(hipMemsetAsync(g_hist, 0, sizeof(float) * np * (size_t)tp->n_history, ctx->stream));
-      if (!shells || !g_sfr) return MRX_OK;
+      if (!shells || !g_sfr) return finish_means();
       MRX_TRY(ensure_buf(ctx, ctx->adj_work, ctx->adj_work_cap, 2 * np));
       float* gshell = ctx->adj_work;
       float* tmp = ctx->adj_work + np;
@@ -1629,7 +1660,7 @@ struct SnapAdj {
           MRX_TRY(check_launch(ctx, "k_sfr_op_adjoint"));
         }
       }
-      return MRX_OK;
+      return finish_means();
     }
   };
 };
