@@ -123,6 +123,12 @@ struct mrx_ctx {
   int ts_shells_N = 0, ts_shells_n = 0;  // grid and shell count ts_shells holds (0: none)
   float* wtab = nullptr;  // window tables of the current radii, (N/2+1)^3 floats each
   size_t wtab_cap = 0;
+  // the bubble ladder's window tables of every filtered radius, built on a
+  // stream of their own (default priority) while the fields are transformed
+  float* wtab_all = nullptr;
+  size_t wtab_all_cap = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t side_ev[2] = {nullptr, nullptr};
   // the bubble ladder's inverse x pass covers ladder_radii radii per launch
   // (tabulated windows): their x-pass outputs, ladder_radii x nfield spectra
   int ladder_radii = kLadderRadii;
@@ -572,21 +578,22 @@ struct Pipe {
   // (not sharp-k): one x pass reads U once and writes the K spectra TK + k
   // tstride, then each radius's y pass runs in place.  Per radius the spectra
   // are inverse_xy's.
+  // wpre: the K radii's window tables already built (window_tables), or null
   static int inverse_xy_k(mrx_ctx* ctx, const float2* U, const float2* V, float2* TK, long tstride, int nfield,
-                          const WinArgs* ws, int K)
+                          const WinArgs* ws, int K, const float* wpre = nullptr)
   {
     constexpr int TW = Cfg<N>::TW;
     const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
-    MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt * K));
-    for (int k = 0; k < K; ++k) {
-      PROF("window_table");
-      k_window_table<N, TW><<<window_table_blocks<N, TW>(), kNT, 0, ctx->stream>>>(ctx->wtab + k * nt, ws[k]);
+    const float* wt = wpre;
+    if (!wt) {
+      MRX_TRY(ensure_buf(ctx, ctx->wtab, ctx->wtab_cap, (size_t)nt * K));
+      MRX_TRY(window_tables(ctx, ctx->wtab, ws, K));
+      wt = ctx->wtab;
     }
-    MRX_TRY(check_launch(ctx, "k_window_table"));
     {
       PROF("ix_filter_k");
       k_ix_filter_k<N><<<N * (NH / TW) * K * nfield, Cfg<N>::COL_NT, 0, ctx->stream>>>(
-        U, V, TK, ctx->tw, ctx->wtab, nt, nc, (long)N * N, tstride, nfield, K);
+        U, V, TK, ctx->tw, wt, nt, nc, (long)N * N, tstride, nfield, K);
     }
     MRX_TRY(check_launch(ctx, "k_ix_filter_k"));
     for (int k = 0; k < K; ++k) {
@@ -594,6 +601,19 @@ struct Pipe {
       colpass_y<true>(ctx, TK + k * tstride, nfield, 0.0);
     }
     return check_launch(ctx, "k_colpass y inv");
+  }
+
+  // the window tables of K radii into Wt (k_window_table's layout, one table
+  // per (N/2+1)^3 floats), on ctx->stream
+  static int window_tables(mrx_ctx* ctx, float* Wt, const WinArgs* ws, int K)
+  {
+    constexpr int TW = Cfg<N>::TW;
+    const long nt = (long)(NH + 1) * (NH + 1) * (NH + 1);
+    for (int k = 0; k < K; ++k) {
+      PROF("window_table");
+      k_window_table<N, TW><<<window_table_blocks<N, TW>(), kNT, 0, ctx->stream>>>(Wt + k * nt, ws[k]);
+    }
+    return check_launch(ctx, "k_window_table");
   }
 
   static int c2r_z(mrx_ctx* ctx, const float2* T, float* out, int nfield, int clamp, int pitch = N + 2,
@@ -800,12 +820,50 @@ struct HiiRun {
       const long fs = (long)npad(N);
       const float* deltax = in[0];
       const float* src[5] = {in[0], in[1], in[2], p->include_spin_temp ? in[3] : in[4], in[4]};
+      const double box_size = p->box_size;
+      // the tabulated windows (top-hat, Gaussian) of every filtered radius
+      // are built on the side stream while the fields are transformed: the
+      // table kernel is bound by its FP64 sincos, the transforms by memory
+      // (up to 4 GiB of tables; beyond that each x pass builds its own)
+      const long nt = (long)(N / 2 + 1) * (N / 2 + 1) * (N / 2 + 1);
+      const float* wpre = nullptr;
+      if (p->filter_type != MRX_FILTER_SHARPK && ctx->ladder_radii >= 1 && S > 1 &&
+          (size_t)nt * (S - 1) * sizeof(float) <= ((size_t)4 << 30) &&
+          ensure_buf(ctx, ctx->wtab_all, ctx->wtab_all_cap, (size_t)nt * (S - 1)) == MRX_OK) {
+        if (!ctx->side) HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        for (auto& e : ctx->side_ev)
+          if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        std::vector<WinArgs> ws(S - 1);
+        for (int k = 0; k < S - 1; ++k) ws[k] = win_args(box_size, (float)R[k], p->filter_type, 1);
+        HIP_TRY(hipEventRecord(ctx->side_ev[0], ctx->stream));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->side_ev[0], 0));
+        hipStream_t const main_stream = ctx->stream;
+        ctx->stream = ctx->side;
+        const int rc = P::window_tables(ctx, ctx->wtab_all, ws.data(), S - 1);
+        ctx->stream = main_stream;
+        MRX_TRY(rc);
+        HIP_TRY(hipEventRecord(ctx->side_ev[1], ctx->side));
+        wpre = ctx->wtab_all;
+      }
+      // the main stream waits for the tables on every exit from here, so no
+      // later call can reuse their buffer while they are being written
+      struct WaitSide {
+        mrx_ctx* c;
+        bool on;
+        ~WaitSide()
+        {
+          if (on) (void)hipStreamWaitEvent(c->stream, c->side_ev[1], 0);
+        }
+      } wait_side{ctx, wpre != nullptr};
       for (int f = 0; f < nfield; ++f) {
         // forward transform of each field into its U slot
         MRX_TRY(P::forward(ctx, src[f], 0, 1, true, ctx->U + (long)f * P::nc, ctx->V + (long)f * N * N));
       }
+      if (wpre) {
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->side_ev[1], 0));
+        wait_side.on = false;
+      }
       (void)fs;
-      const double box_size = p->box_size;
       const double pixel_volume = pow(box_size / (double)N, 3);
       StepArgs a;
       a.pixel_volume = pixel_volume;
@@ -831,7 +889,7 @@ struct HiiRun {
           WinArgs ws[kLadderRadiiMax];
           for (int k = 0; k < K; ++k) ws[k] = win_args(box_size, (float)R[s + k], p->filter_type, 1);
           float2* TK = KL > 1 ? ctx->TK : ctx->T;
-          MRX_TRY(P::inverse_xy_k(ctx, ctx->U, ctx->V, TK, tstride, nfield, ws, K));
+          MRX_TRY(P::inverse_xy_k(ctx, ctx->U, ctx->V, TK, tstride, nfield, ws, K, wpre ? wpre + s * nt : nullptr));
           a.kzlive = N / 2;
         }
         for (int k = 0; k < K; ++k, ++s) {
@@ -1829,6 +1887,13 @@ int mrx_free(mrx_ctx* ctx)
   dfree(ctx->ts_work);
   dfree(ctx->ts_shells);
   dfree(ctx->wtab);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  for (auto& e : ctx->side_ev)
+    if (e) (void)hipEventDestroy(e);
+  dfree(ctx->wtab_all);
   if (ctx->TK) (void)hipFree(ctx->TK);
   dfree(ctx->adj_work);
   dfree(ctx->adj_gS);
