@@ -281,19 +281,23 @@ __device__ __forceinline__ float lds_real(const float2* s, int e)
   return (e & 1) ? v.y : v.x;
 }
 
-// aliasing clamps of find_HII_bubbles.c:272-311
+// aliasing clamps of find_HII_bubbles.c:272-311.  Kinds 0-2 are written as
+// compares and selects with fmaxf/fminf's results for every input, NaN
+// included (fmaxf(NaN, c) = c, and a compare with NaN is false): a max of a
+// value loaded from memory costs a canonicalising max before it, a select
+// does not.  Kind 1 is max(v, 0) then v < ABS_TOL -> 0, which is v >= ABS_TOL
+// ? v : 0.
 __device__ __forceinline__ float clamp_kind(float v, int kind)
 {
   switch (kind) {
-    case 0:
-      return fmaxf(v, -1.0f + kRelTol);
+    case 0: {
+      constexpr float lo = -1.0f + kRelTol;
+      return (v > lo) ? v : lo;
+    }
     case 1:
-      v = fmaxf(v, 0.0f);
-      return (v < kAbsTol) ? 0.0f : v;
+      return (v >= kAbsTol) ? v : 0.0f;
     case 2:
-      v = fmaxf(v, 0.0f);
-      v = (v < kAbsTol) ? 0.0f : v;
-      return fminf(v, 0.999f);
+      return (v >= kAbsTol) ? ((v < 0.999f) ? v : 0.999f) : 0.0f;
     case 3:  // spin-temperature shells: max(v, 0) only (ComputeTs.c:417, :488)
       return fmaxf(v, 0.0f);
     default:
@@ -947,14 +951,16 @@ __device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, 
 #pragma clang fp contract(off)
   constexpr bool XE = (NF - (REC ? 1 : 0)) == 4;
   constexpr int RI = XE ? 4 : 3;  // field of N_rec
-  const double density_over_mean = 1.0 + (double)v[0];
-  const double neutral_fraction = XE ? 1.0 - (double)v[3] : 1.0;
+  // the double quantities are formed in the branches that use them: the
+  // single-precision test below decides almost every cell without them
+  auto density_over_mean = [&]() { return 1.0 + (double)v[0]; };
+  auto neutral_fraction = [&]() { return XE ? 1.0 - (double)v[3] : 1.0; };
   // recombinations in the cell (:360-363)
-  const double rec = REC ? (double)v[RI] / density_over_mean : 0.0;
+  auto rec = [&]() { return REC ? (double)v[RI] / density_over_mean() : 0.0; };
   // the reference's f_coll (four roundings of its exact value), evaluated
   // where needed: the last step's partial xH and near-threshold cells
   auto f_coll = [&]() {
-    return (double)v[1] / (a.M_mean * density_over_mean) * (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume;
+    return (double)v[1] / (a.M_mean * density_over_mean()) * (4.0 / 3.0) * M_PI * a.R_cubed / a.pixel_volume;
   };
   // the test f_coll*eff > n_f as stars*kfc*eff vs n_f*(1+delta), both
   // positive sides within 1e-14 of their exact values: a relative gap
@@ -976,16 +982,19 @@ __device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, 
     }
   }
   if (!decided) {
-    const double lhs = (double)v[1] * a.kfc * a.eff, rhs = neutral_fraction * (1. + rec) * density_over_mean;
+    const double lhs = (double)v[1] * a.kfc * a.eff, rhs = neutral_fraction() * (1. + rec()) * density_over_mean();
     if (lhs > rhs * (1.0 + 1e-12))
       cross = true;
     else if (lhs < rhs * (1.0 - 1e-12))
       cross = false;
     else
-      cross = f_coll() * a.eff > neutral_fraction * (1. + rec);
+      cross = f_coll() * a.eff > neutral_fraction() * (1. + rec());
   }
+  // the weighted SFR is clamped here, where it is used (the clamp is
+  // idempotent, so a caller may pass it clamped or not)
+  auto sfr = [&]() { return clamp_kind(v[2], 1); };
   // J_21 of the cell at this radius (:317-320, :331-337)
-  auto j21_aux = [&]() { return a.uvb ? (float)((double)v[2] / a.pixel_volume * a.J21c) : 0.0f; };
+  auto j21_aux = [&]() { return a.uvb ? (float)((double)sfr() / a.pixel_volume * a.J21c) : 0.0f; };
   const bool never = (st == 0xFFu);
   zcheck = false;
   jaux = 0.0f;
@@ -993,7 +1002,7 @@ __device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, 
   if (cross) {
     if (never) {
       if (a.uvb) J21[ir] = j21_aux();
-      if (REC) Gamma12[ir] = (float)(a.gamma_pref * ((double)v[2] / a.pixel_volume));  // :398-404
+      if (REC) Gamma12[ir] = (float)(a.gamma_pref * ((double)sfr() / a.pixel_volume));  // :398-404
     }
     st_new = (uint32_t)a.step;
   }
@@ -1002,7 +1011,7 @@ __device__ __forceinline__ uint32_t threshold_cell(const float* v, uint32_t st, 
     if (ion) {
       xh = 0.0f;
     } else {
-      xh = (float)(neutral_fraction - f_coll() * a.eff);
+      xh = (float)(neutral_fraction() - f_coll() * a.eff);
       if (xh < 0.f)
         xh = 0.f;
       else if (xh > 1.0f)
@@ -1068,8 +1077,9 @@ __device__ __forceinline__ void c2r_threshold_tile(const float2* __restrict__ T,
   float val[NF][CPT];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    // clamps of :272-311: deltax, x_e, the rest (stars, sfr, N_rec)
-    const int kind = (f == 0) ? 0 : ((XE && f == 3) ? 2 : 1);
+    // clamps of :272-311: deltax, x_e, the rest (stars, N_rec); the weighted
+    // SFR is clamped in threshold_cell, at its uses
+    const int kind = (f == 0) ? 0 : (f == 2 ? -1 : ((XE && f == 3) ? 2 : 1));
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       const int h = f * ZE + 2 * (tid + j * NT);  // float2 index of the group's first pair
@@ -1323,7 +1333,8 @@ __global__ __launch_bounds__(256) void k_c2r_threshold_reg(const float2* __restr
     float v[2][NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      const int kind = (f == 0) ? 0 : ((XE && f == 3) ? 2 : 1);  // clamps of :272-311
+      // clamps of :272-311; the weighted SFR's is threshold_cell's
+      const int kind = (f == 0) ? 0 : (f == 2 ? -1 : ((XE && f == 3) ? 2 : 1));
       const float2 p = xb[rl][f][j * XP + t];
       v[0][f] = clamp_kind(p.x, kind);
       v[1][f] = clamp_kind(p.y, kind);
