@@ -121,13 +121,15 @@ __device__ __forceinline__ D7 ts_cell_dual(const TsDev& t, const double* tabs, f
   }
   const D7 sink = alpha_A_t(TD) * 2.0 * xD * xD * t.f_H * n_b;
   const D7 dxe_dzp = t.dt_dzp * (dxion - sink);
+  // ts_cell_update's products with reciprocals, in its order
+  const D7 inv1x = 1.0 / (1.0 + xD);
   D7 dadia(t.dadia0);
-  if (fabsf(dl) > 1e-7) dadia += t.dgrowth / (1.0 / dlD + t.growth);
+  if (fabsf(dl) > 1e-7) dadia += t.dgrowth * dlD / (1.0 + t.growth * dlD);
   dadia *= (2.0 / 3.0) * TD;
-  const D7 dspec = -dxe_dzp * TD / (1.0 + xD);
-  D7 dcomp = (-1.51e-4) * (xD / (1.0 + xD + t.f_He)) / t.HH / t.hubble_h * t.T4 / t.one_p_z;
+  const D7 dspec = -dxe_dzp * TD * inv1x;
+  D7 dcomp = (-1.51e-4) * (xD / (1.0 + xD + t.f_He)) * t.comp;
   dcomp *= t.Trad - TD;
-  const D7 dxheat_dzp = dxheat * t.dt_dzp * 2.0 / 3.0 / 1.3806e-16 / (1.0 + xD);
+  const D7 dxheat_dzp = dxheat * t.dt_dzp * kHeatFac * inv1x;
   const D7 der1 = dxheat_dzp + dcomp + dspec + dadia;
   const D7 der2 = dxlya + dstarlya;
   D7 xn = rnd_f(xD + dxe_dzp * t.dzp);

@@ -1082,6 +1082,8 @@ void fill_ts_dev(TsDev& t, const mrx_ts_data* d, const mrx_ts_snap* s, const mrx
   t.HH = H / (3.2407789e-18 * p->hubble_h);
   t.T4 = pow(t.Trad, 4.0);
   t.one_p_z = 1.0 + zp;
+  t.invTrad = 1.0 / t.Trad;
+  t.comp = t.T4 / (t.HH * p->hubble_h * t.one_p_z);
   t.Tk_floor = (float)(2.728 * (1 + s->zp));
   t.No = d->No;
   t.N_b0 = d->N_b0;

@@ -67,6 +67,8 @@ struct TsDev {
   double XA;        // 1.66e11 / (1.0 + zp_f)
   double HH;        // hubble((float)zp) / (HUBBLE h)
   double T4;        // pow(Trad, 4.0)
+  double invTrad;   // 1 / Trad
+  double comp;      // T4 / (HH hubble_h one_p_z): the Compton term's snapshot factor
   double one_p_z;   // 1.0 + zp_f
   double Tk_floor;  // (float)(TCMB (1 + zp)) with zp double
   double No, N_b0, f_H, f_He, hubble_h;
@@ -169,53 +171,57 @@ __device__ __forceinline__ R kappa_file(const KappaSpline& s, R TK)
 }
 
 // get_Ts (XRayHeatingFunctions.c:1513-1558) at this snapshot's z, on the
-// float inputs widened to R; the caller rounds the result to float.
+// float inputs widened to R; the caller rounds the result to float.  Its
+// divisions by T_K, T_S, T_rad, Sden and 2.85e-15 are products with
+// reciprocals formed once (each within two ulp of the quotient, far below the
+// float the result is rounded to): three divisions before the fixed point and
+// two per iteration, against eleven and four.
 template <class R>
 __device__ __forceinline__ R get_Ts_t(const TsDev& t, R dl, R TK, R xe, R Jalpha, R* xa_out)
 {
-  const double Trad = t.Trad;
+  constexpr double kInvA10 = 1.0 / 2.85e-15;
+  const double invTrad = t.invTrad;
+  const R invTK = 1.0 / TK;
   // xcoll = xcoll_HI + xcoll_elec + xcoll_prot (:1560-1600)
   const R nH = (1.0 - xe) * t.No * t.P3 * (1.0 + dl);
-  const R xHI = t.T21_Trad * nH * kappa_HH(t.kHH, TK) / 2.85e-15;
+  const R xHI = t.T21_Trad * nH * kappa_HH(t.kHH, TK) * kInvA10;
   const R ne = xe * t.N_b0 * t.P3 * (1.0 + dl);
-  const R xel = t.T21_Trad * ne * kappa_file(t.keH, TK) / 2.85e-15;
+  const R xel = t.T21_Trad * ne * kappa_file(t.keH, TK) * kInvA10;
   const R np = xe * t.No * t.P3 * (1.0 + dl);
-  const R xpr = t.T21_Trad * np * kappa_file(t.kpH, TK) / 2.85e-15;
+  const R xpr = t.T21_Trad * np * kappa_file(t.kpH, TK) * kInvA10;
   const R xc = xHI + xel + xpr;
   R TS;
   if (pval(Jalpha) > 1.0e-20) {
     R TSold(0.0), xa(0.0);
-    TS = R(Trad);
+    TS = R(t.Trad);
+    R invTS = R(invTrad);
     const R tgp = t.TGP * (1.0 + dl) * (1.0 - xe);
-    // the terms of the iteration that do not involve TS, evaluated once (the
-    // same operations in the same order as the reference's loop body)
+    // the terms of the iteration that do not involve TS, evaluated once
     // the cube root in single precision (cbrtf, within 1 ulp): xi is a
     // correction to S of a few per cent, so its float rounding moves T_S by
     // well under 1e-7
-    const R xarg = 1.0e-7 * tgp / TK / TK;
+    const R xarg = 1.0e-7 * tgp * invTK * invTK;
     const double xa0 = pval(xarg), xif = (double)cbrtf((float)xa0);
     const R xi = chain(xarg, xif, xa0 > 0.0 ? xif / (3.0 * xa0) : 0.0);
-    const R S0 = 1.0 - 0.0631789 / TK + 0.115995 / TK / TK;
+    const R S0 = 1.0 - 0.0631789 * invTK + 0.115995 * invTK * invTK;
     const R Sden = 1.0 + 2.98394 * xi + 1.53583 * xi * xi + 3.85289 * xi * xi * xi;
-    const R invTK = 1.0 / TK, ceTK = 0.405535 / TK, xcTK = xc / TK;
-    const double invTrad = 1.0 / Trad;
-    // the loop body's divisions by TS, TK and Sden as products with their
-    // reciprocals, and xa / Tceff as xa * ce (Tceff = 1 / ce): three divisions
-    // per iteration instead of ten, the same values to rounding
+    const R ceTK = 0.405535 * invTK, xcTK = xc * invTK;
+    // xa / Tceff as xa * ce (Tceff = 1 / ce); 1 / TS of the next iteration is
+    // this one's TSinv
     const R aTK = 0.401403 * invTK, bTK = 0.336463 * invTK * invTK, invSden = 1.0 / Sden;
     const R XJ = t.XA * Jalpha;
-    while (fabs(pval(TS) - pval(TSold)) / pval(TS) > 1.0e-3) {
+    // the reference's test |TS - TSold| / TS > 1e-3 (false for TS <= 0 or NaN)
+    while (pval(TS) > 0.0 && fabs(pval(TS) - pval(TSold)) > 1.0e-3 * pval(TS)) {
       TSold = TS;
-      const R invTS = 1.0 / TS;
       const R S = (S0 - aTK * invTS + bTK * invTS) * invSden;
       xa = XJ * S;
       const R ce = invTK + ceTK * (invTS - invTK);
-      const R TSinv = (invTrad + xa * ce + xcTK) / (1.0 + xa + xc);
-      TS = 1.0 / TSinv;
+      invTS = (invTrad + xa * ce + xcTK) / (1.0 + xa + xc);
+      TS = 1.0 / invTS;
     }
     *xa_out = xa;
   } else {
-    const R TSinv = (1.0 / Trad + xc / TK) / (1.0 + xc);
+    const R TSinv = (invTrad + xc * invTK) / (1.0 + xc);
     TS = 1.0 / TSinv;
     *xa_out = R(0.0);
   }
@@ -236,7 +242,7 @@ __device__ __forceinline__ float dev_get_Ts(const TsDev& t, float delta, float T
 template <class R>
 __device__ __forceinline__ R alpha_A_t(R T)
 {
-  const R logT = log(T / (double)1.1604505e4);
+  const R logT = log(T * (1.0 / (double)1.1604505e4));
   // the reference's pow(logT, k) as products (each within a few ulp of pow,
   // which is itself not correctly rounded); the sum keeps its order
   const R p2 = logT * logT, p3 = p2 * logT, p4 = p2 * p2, p5 = p4 * logT, p6 = p3 * p3, p7 = p6 * logT,
@@ -247,6 +253,9 @@ __device__ __forceinline__ R alpha_A_t(R T)
 }
 
 __device__ __forceinline__ double dev_alpha_A(double T) { return alpha_A_t<double>(T); }
+
+// 2 / 3 / k_B of evolveInt's heating term (XRayHeatingFunctions.c:1241-1435)
+constexpr double kHeatFac = 2.0 / 3.0 / 1.3806e-16;
 
 // Per-cell spin-temperature update (ComputeTs.c:743-900); 7 block partial sums:
 // J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e.
@@ -366,13 +375,16 @@ __device__ __forceinline__ void ts_cell_update(const TsDev& t, const ShellSums& 
   }
   const double sink = dev_alpha_A(T) * 2.0 * x_e * x_e * t.f_H * n_b;
   const double dxe_dzp = t.dt_dzp * (dxion - sink);
+  // the divisions by 1 + x_e, by the snapshot constants and by 1/dl + growth
+  // as products (each within two ulp; x_e and T_K are rounded to float)
+  const double inv1x = 1.0 / (1.0 + x_e);
   double dadia = t.dadia0;
-  if (fabs(dl) > 1e-7) dadia += t.dgrowth / (1.0 / dl + t.growth);
+  if (fabs(dl) > 1e-7) dadia += t.dgrowth * dl / (1.0 + t.growth * dl);
   dadia *= (2.0 / 3.0) * T;
-  const double dspec = -dxe_dzp * T / (1 + x_e);
-  double dcomp = (-1.51e-4) * (x_e / (1.0 + x_e + t.f_He)) / t.HH / t.hubble_h * t.T4 / t.one_p_z;
+  const double dspec = -dxe_dzp * T * inv1x;
+  double dcomp = (-1.51e-4) * (x_e / (1.0 + x_e + t.f_He)) * t.comp;
   dcomp *= t.Trad - T;
-  const double dxheat_dzp = dxheat * t.dt_dzp * 2.0 / 3.0 / 1.3806e-16 / (1.0 + x_e);
+  const double dxheat_dzp = dxheat * t.dt_dzp * kHeatFac * inv1x;
   const double der1 = dxheat_dzp + dcomp + dspec + dadia;
   const double der2 = dxlya + dstarlya;
   const double der4 = t.dt_dzp * dxion;
