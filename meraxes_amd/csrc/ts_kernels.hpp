@@ -12,8 +12,10 @@
 // The per-cell kernel is ComputeTs.c:743-900 with evolveInt
 // (XRayHeatingFunctions.c:1241-1435) and get_Ts (:1513-1558) in FP64; every
 // cell-independent sub-expression that the reference evaluates first (a
-// left-to-right prefix) is evaluated once on the host and passed in, so the
-// device repeats the reference's own operation order.
+// left-to-right prefix) is evaluated once on the host and passed in, and the
+// reference's divisions by T_K, T_S, 1 + x_e and snapshot constants are
+// products with reciprocals (within two ulp of each quotient; x_e, T_K and
+// T_S are rounded to float).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
