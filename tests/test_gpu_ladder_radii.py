@@ -4,7 +4,7 @@ With tabulated windows (top-hat, Gaussian) the inverse x pass covers several
 radii per launch (MRX_LADDER_RADII, read when a context is created, 2 by
 default).  Each radius's spectra are those of the one-radius pass, so every
 output grid and global must be identical for 1, 3 (groups that do not divide
-the ladder) and 4 radii per pass.
+the ladder) and 4 radii per pass, from 64^3 to the full 512^3.
 """
 import os
 
@@ -34,7 +34,7 @@ def run_with(radii, p, d, s, w, xe):
         c.close()
 
 
-@pytest.mark.parametrize("N,filt,spin", [(64, 0, 0), (64, 2, 1), (128, 0, 1)])
+@pytest.mark.parametrize("N,filt,spin", [(64, 0, 0), (64, 2, 1), (128, 0, 1), (512, 0, 1)])
 def test_radii_per_pass_do_not_change_results(ctx, N, filt, spin):
     print(f"ladder radii N={N} filter={filt} spin={spin}", flush=True)
     d, s, w = synth.snapshot_inputs(N, 17)
