@@ -841,8 +841,13 @@ struct HiiRun {
         ctx->stream = ctx->side;
         const int rc = P::window_tables(ctx, ctx->wtab_all, ws.data(), S - 1);
         ctx->stream = main_stream;
-        MRX_TRY(rc);
-        HIP_TRY(hipEventRecord(ctx->side_ev[1], ctx->side));
+        const hipError_t er = (rc == MRX_OK) ? hipEventRecord(ctx->side_ev[1], ctx->side) : hipSuccess;
+        if (rc != MRX_OK || er != hipSuccess) {
+          // the tables already launched finish before the buffer can be reused
+          (void)hipStreamSynchronize(ctx->side);
+          if (rc != MRX_OK) return rc;
+          return set_err(ctx, MRX_EDEVICE, "hipEventRecord (window tables) failed: %s", hipGetErrorString(er));
+        }
         wpre = ctx->wtab_all;
       }
       // the main stream waits for the tables on every exit from here, so no
