@@ -141,7 +141,7 @@ __device__ __forceinline__ D7 ts_cell_dual(const TsDev& t, const double* tabs, f
   if (pval(Tn) < (double)(float)5e4) Tn = rnd_f(Tn + der1 * t.dzp);
   if (pval(Tn) < 0) Tn = D7((double)(float)t.Tk_floor);
   D7 xa;
-  const D7 ts = rnd_f(get_Ts_t<D7>(t, dlD, Tn, xn, rnd_f(der2), &xa));
+  const D7 ts = rnd_f(get_Ts_t<D7>(t, &t.kHH, dlD, Tn, xn, rnd_f(der2), &xa));
   *xn_out = xn;
   return ts;
 }
@@ -208,7 +208,8 @@ __global__ __launch_bounds__(kNT) void k_snapshot_adjoint_cell(
   } else if (a.ts_mode == 1) {
     using D1 = Dual<1>;
     D1 xa;
-    const D1 tsd = get_Ts_t<D1>(t, D1::var((double)dl, 0), D1((double)a.h_T0), D1((double)a.h_xe0), D1(0.0), &xa);
+    const D1 tsd = get_Ts_t<D1>(t, &t.kHH, D1::var((double)dl, 0), D1((double)a.h_T0), D1((double)a.h_xe0), D1(0.0),
+                                &xa);
     gdl += gTS * tsd.d[0];
   }
   g_delta[ip] = (float)gdl;

@@ -18,6 +18,7 @@
 // T_S are rounded to float).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "reion_kernels.hpp"
@@ -178,19 +179,21 @@ __device__ __forceinline__ R kappa_file(const KappaSpline& s, R TK)
 // reciprocals formed once (each within two ulp of the quotient, far below the
 // float the result is rounded to): three divisions before the fixed point and
 // two per iteration, against eleven and four.
+// ks: the three collision-rate splines kHH, keH, kpH (t's own, or a copy of
+// them in LDS, where the per-lane knot searches are cheaper)
 template <class R>
-__device__ __forceinline__ R get_Ts_t(const TsDev& t, R dl, R TK, R xe, R Jalpha, R* xa_out)
+__device__ __forceinline__ R get_Ts_t(const TsDev& t, const KappaSpline* ks, R dl, R TK, R xe, R Jalpha, R* xa_out)
 {
   constexpr double kInvA10 = 1.0 / 2.85e-15;
   const double invTrad = t.invTrad;
   const R invTK = 1.0 / TK;
   // xcoll = xcoll_HI + xcoll_elec + xcoll_prot (:1560-1600)
   const R nH = (1.0 - xe) * t.No * t.P3 * (1.0 + dl);
-  const R xHI = t.T21_Trad * nH * kappa_HH(t.kHH, TK) * kInvA10;
+  const R xHI = t.T21_Trad * nH * kappa_HH(ks[0], TK) * kInvA10;
   const R ne = xe * t.N_b0 * t.P3 * (1.0 + dl);
-  const R xel = t.T21_Trad * ne * kappa_file(t.keH, TK) * kInvA10;
+  const R xel = t.T21_Trad * ne * kappa_file(ks[1], TK) * kInvA10;
   const R np = xe * t.No * t.P3 * (1.0 + dl);
-  const R xpr = t.T21_Trad * np * kappa_file(t.kpH, TK) * kInvA10;
+  const R xpr = t.T21_Trad * np * kappa_file(ks[2], TK) * kInvA10;
   const R xc = xHI + xel + xpr;
   R TS;
   if (pval(Jalpha) > 1.0e-20) {
@@ -232,11 +235,11 @@ __device__ __forceinline__ R get_Ts_t(const TsDev& t, R dl, R TK, R xe, R Jalpha
 }
 
 // float in, float out, as the reference calls it
-__device__ __forceinline__ float dev_get_Ts(const TsDev& t, float delta, float TK_f, float xe_f, float Jalpha,
-                                           float* xa_out)
+__device__ __forceinline__ float dev_get_Ts(const TsDev& t, const KappaSpline* ks, float delta, float TK_f,
+                                           float xe_f, float Jalpha, float* xa_out)
 {
   double xa;
-  const double TS = get_Ts_t<double>(t, (double)delta, (double)TK_f, (double)xe_f, (double)Jalpha, &xa);
+  const double TS = get_Ts_t<double>(t, ks, (double)delta, (double)TK_f, (double)xe_f, (double)Jalpha, &xa);
   *xa_out = (float)xa;
   return (float)TS;
 }
@@ -287,6 +290,19 @@ __device__ __forceinline__ void load_ts_coef(const TsDev& t, double* tabs)
     const int k = u * kTsCoefLanes + (int)threadIdx.x;
     if (k < n2) dst[k] = v[u];
   }
+}
+
+// Copies the three collision-rate splines (kHH, keH, kpH, adjacent in TsDev)
+// into LDS in 4-byte words over the workgroup's lanes.
+__device__ __forceinline__ void load_kappa(const TsDev& t, KappaSpline* ks)
+{
+  constexpr int kWords = 3 * (int)sizeof(KappaSpline) / 4;
+  static_assert(offsetof(TsDev, keH) == offsetof(TsDev, kHH) + sizeof(KappaSpline) &&
+                  offsetof(TsDev, kpH) == offsetof(TsDev, keH) + sizeof(KappaSpline),
+                "the splines are adjacent");
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&t.kHH);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(ks);
+  for (int w = (int)threadIdx.x; w < kWords; w += (int)blockDim.x) dst[w] = src[w];
 }
 
 // The cell's seven shell sums at its x_e bracket `lo` (ComputeTs.c:759-790,
@@ -362,9 +378,9 @@ __device__ __forceinline__ int ts_bracket(const TsDev& t, float xe0, double* fr)
 // when no_light): the new x_e, T_K and T_S, and the cell's terms of the seven
 // block sums (J_alpha, x_alpha, Xheat, Xion, T_S, T_K, x_e).  dl, xe0, T: the
 // cell's delta, previous x_e and T_K; fr: its x_e bracket offset.
-__device__ __forceinline__ void ts_cell_update(const TsDev& t, const ShellSums& S, float dl, float xe0, double T,
-                                               double fr, float& xn_out, float& Tn_out, float& ts_out,
-                                               double (&sum)[7])
+__device__ __forceinline__ void ts_cell_update(const TsDev& t, const KappaSpline* ks, const ShellSums& S, float dl,
+                                               float xe0, double T, double fr, float& xn_out, float& Tn_out,
+                                               float& ts_out, double (&sum)[7])
 {
   const double x_e = xe0;
   const double n_b = t.NBP3 * (1 + dl);
@@ -399,7 +415,7 @@ __device__ __forceinline__ void ts_cell_update(const TsDev& t, const ShellSums& 
   if (Tn < (float)5e4) Tn = (float)((double)Tn + der1 * t.dzp);
   if (Tn < 0) Tn = (float)t.Tk_floor;
   float xa;
-  const float ts = dev_get_Ts(t, dl, Tn, xn, (float)der2, &xa);
+  const float ts = dev_get_Ts(t, ks, dl, Tn, xn, (float)der2, &xa);
   xn_out = xn;
   Tn_out = Tn;
   ts_out = ts;
@@ -421,8 +437,10 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
 {
   extern __shared__ double tabs[];
   __shared__ double red[7 * kNT / 64];
+  __shared__ KappaSpline ks[3];
   const TsDev& t = *tg;
   load_ts_coef(t, tabs);
+  load_kappa(t, ks);
   __syncthreads();
   const long ncell = (long)N * N * N;
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -439,7 +457,7 @@ __global__ __launch_bounds__(kNT) void k_ts_cell(const TsDev* __restrict__ tg, c
     ShellSums S = {0, 0, 0, 0, 0, 0, 0};
     if (!t.no_light) S = ts_shell_sums(t, tabs, lo, shells, ncell, i);
     float xn, Tn, ts;
-    ts_cell_update(t, S, dl, xe0, T, fr, xn, Tn, ts, v);
+    ts_cell_update(t, ks, S, dl, xe0, T, fr, xn, Tn, ts, v);
     xe_prev[ip] = xn;
     Tk[i] = Tn;
     TS[i] = ts;
@@ -468,7 +486,7 @@ __global__ __launch_bounds__(kNT) void k_ts_homogeneous(const TsDev* __restrict_
     const int z = (int)(i % N);
     const long ip = r * (N + 2) + z;
     float xa;
-    const float ts = dev_get_Ts(t, deltax[ip], T0, xe0, 0.0f, &xa);
+    const float ts = dev_get_Ts(t, &t.kHH, deltax[ip], T0, xe0, 0.0f, &xa);
     xe_prev[ip] = xe0;
     Tk[i] = T0;
     TS[i] = ts;
