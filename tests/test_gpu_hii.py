@@ -64,7 +64,7 @@ def check_parity(g, o, N, label, eff=20.0, neutral_vs_fp64=False):
     neutral = keep & ~ion
     # partial xH = n_f - eff * f_coll on the unfiltered last step: the float
     # transforms' ~1e-7 (of the field's rms) error is multiplied by eff (larger
-    # boxes judge these cells against the FP64 oracle instead: neutral_vs_fp64)
+    # boxes check these cells against the FP64 oracle instead: neutral_vs_fp64)
     if not neutral_vs_fp64:
         assert np.allclose(g["xH"][neutral], o["xH"][neutral], rtol=0, atol=2.5e-6 * eff)
     zi = keep & (o["z_at_ionization"] >= 0)
